@@ -1,0 +1,240 @@
+"""Pressure-solve benchmark (one MI355X).
+
+Workload (default): the 16.8M-cell stretched box -- 256^3 hex cells, 30
+wall-clustered layers at each y wall (first-cell aspect ratio 1000), interior
+vertices jittered +-10 %, cells randomly permuted, log-normal 1/a_P
+(sigma 1.5, correlation 8 cells), all-Neumann pressure system with the
+reference cell eliminated, zero-mean N(0,1) right-hand side. The matrix is
+assembled on the GPU by the face-loop kernels.
+
+A step is one complete solve from x = 0 to ||b - A x||_2 / ||b||_2 <= 1e-8
+(FP64 true residual recomputed on the GPU):
+  * value:  FP64 FGMRES(30) outer iteration with the FP32 smoothed-aggregation
+            V-cycle as preconditioner (mixed precision),
+  * plain:  FP64 AMG-PCG with the FP64 V-cycle (the reference algorithm),
+both on the same GPU and the same hierarchy construction. Setup is timed
+separately and is not part of a step.
+
+The JSON line also carries the roofline of the dominant kernel (the CSR-stream
+SpMV family; measured here as the finest-level FP64 SpMV with HIP events on
+the solver's stream) and the reference's CPU PCG timed on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Solve s to 1e-8 rel residual, learned+mixed vs FP64 AMG-PCG; SpMV TB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, sv, sx):
+    # values + 4 B column per nonzero, 4 B per row offset, x read once, y written once
+    return nnz * (sv + 4) + 4 * (n + 1) + n * sx + n * sx
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(sample_n: int, tol: float):
+    """Reference CPU PCG (oracle/_ref: the reference's own NativeAMG + solve_cg_prepared),
+    or the oracle restatement when the reference build is absent, on the same
+    workload generator at sample_n^3 cells. 1 thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, reference_available
+
+    import cfdx_dev_amd.workloads as W
+
+    s = W.c5_box(n=sample_n)
+    kind = "reference" if reference_available() else "port"
+    o = Oracle(reference=(kind == "reference"))
+    amg = o.amg("sa")
+    t0 = time.perf_counter()
+    ok = amg.setup(s.n, s.rowp, s.cols, s.vals)
+    t_setup = time.perf_counter() - t0
+    if not ok:
+        return None
+    x = np.zeros(s.n)
+    t0 = time.perf_counter()
+    r = amg.pcg(s.b, x, 2000, tol)
+    t_solve = time.perf_counter() - t0
+    return dict(value=t_solve, unit="s", cores=1, kind=kind,
+                sample=f"{s.name}: {sample_n}^3 = {s.n + 1} cells, nnz {len(s.vals)}, FP64 SA-AMG PCG "
+                       f"to {tol:g} from x=0, {r.iterations} iterations, status {r.status}; "
+                       f"setup {t_setup:.2f} s not included" + (" (setup ran the reference's two hierarchy "
+                                                                 "builds)" if kind == "reference" else ""),
+                setup_s=t_setup, iterations=int(r.iterations), cpu=cpu_model(), nproc=os.cpu_count())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256, help="cells per direction of the box")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-sample-n", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-repeat", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(multi-GPU sharding is a design note, not implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    import cfdx_dev_amd as cx
+    import cfdx_dev_amd.workloads as W
+
+    t0 = time.perf_counter()
+    s = W.c5_box(n=args.n)
+    t_build = time.perf_counter() - t0
+    log(f"[bench] workload {s.name}: n={s.n} nnz={len(s.vals)} built in {t_build:.1f} s")
+
+    plain = cx.NativeAMGPreconditioner(method="sa", precision="fp64")
+    t0 = time.perf_counter()
+    assert plain.setup(s.n, s.rowp, s.cols, s.vals), plain.last_error()
+    t_setup_plain = time.perf_counter() - t0
+    hs, ds = plain.setup_seconds()
+    levels = [plain.level_info(l) for l in range(plain.num_levels())]
+    log(f"[bench] FP64 hierarchy: {len(levels)} levels {[(a, b) for a, b, _ in levels]} setup {t_setup_plain:.1f} s "
+        f"(host {hs:.1f} s, device {ds:.1f} s)")
+    mixed = cx.NativeAMGPreconditioner(method="sa", precision="fp32")
+    t0 = time.perf_counter()
+    assert mixed.setup(s.n, s.rowp, s.cols, s.vals), mixed.last_error()
+    t_setup_mixed = time.perf_counter() - t0
+    log(f"[bench] FP32 hierarchy setup {t_setup_mixed:.1f} s")
+
+    def run_plain():
+        x = np.zeros(s.n)
+        r = plain.solve_cg(s.b, x, 1000, args.tol)
+        return r
+
+    def run_mixed():
+        x = np.zeros(s.n)
+        r = mixed.solve_fgmres(s.b, x, 30, 1000, args.tol)
+        return r
+
+    for _ in range(args.warmup):
+        run_plain()
+        run_mixed()
+    tp, tm, rp_last, rm_last = [], [], None, None
+    for k in range(args.steps):
+        rp_last = run_plain()
+        tp.append(rp_last.solve_seconds)
+        rm_last = run_mixed()
+        tm.append(rm_last.solve_seconds)
+        log(f"[bench] step {k}: plain PCG {rp_last.solve_seconds:.3f} s ({rp_last.iterations} it, "
+            f"rel {rp_last.residual_relative:.2e}) | mixed FGMRES {rm_last.solve_seconds:.3f} s "
+            f"({rm_last.iterations} it, rel {rm_last.residual_relative:.2e})")
+    t_plain = float(np.mean(tp))
+    t_mixed = float(np.mean(tm))
+
+    # ---- SpMV roofline (finest level, FP64 and FP32), HIP events on the solver stream
+    import ctypes as C
+
+    lib = cx._native.load()
+    nnz = len(s.vals)
+    from cfdx_dev_amd._native import LIB_PATH  # noqa: F401
+
+    def spmv_time(handle, precision, dtype):
+        x = np.random.default_rng(1).standard_normal(s.n).astype(dtype)
+        # device buffers through the library's own spmv entry (caller-ordered)
+        hip = C.CDLL("libamdhip64.so")
+        dx, dy = C.c_void_p(), C.c_void_p()
+        nb = s.n * np.dtype(dtype).itemsize
+        assert hip.hipMalloc(C.byref(dx), C.c_size_t(nb)) == 0
+        assert hip.hipMalloc(C.byref(dy), C.c_size_t(nb)) == 0
+        hip.hipMemcpy(dx, x.ctypes.data_as(C.c_void_p), C.c_size_t(nb), 1)
+        sec = C.c_double()
+        st = lib.cfdxhip_amg_spmv_device(handle._h, precision, dx, dy, args.spmv_repeat, C.byref(sec))
+        hip.hipFree(dx)
+        hip.hipFree(dy)
+        assert st == 0, handle.last_error()
+        return sec.value
+
+    t64 = spmv_time(plain, 0, np.float64)
+    t32 = spmv_time(mixed, 1, np.float32)
+    b64 = spmv_bytes(s.n, nnz, 8, 8)
+    b32 = spmv_bytes(s.n, nnz, 4, 4)
+    gbs64 = b64 / t64 / 1e9
+    gbs32 = b32 / t32 / 1e9
+    log(f"[bench] SpMV FP64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s | FP32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(args.cpu_sample_n, args.tol)
+        log(f"[bench] CPU baseline ({time.perf_counter() - t0:.1f} s): {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(t_mixed, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 outer / f32 hierarchy",
+        "data": "synthetic",
+        "config": {
+            "workload": s.name,
+            "cells": int(s.n + 1),
+            "unknowns": int(s.n),
+            "nnz": int(nnz),
+            "tolerance": args.tol,
+            "solver": "FGMRES(30) FP64 outer + FP32 SA V-cycle (6+6 damped Jacobi)",
+            "learned_stage": False,
+            "parallelism": "single GPU",
+        },
+        "plain_fp64_pcg_s": round(t_plain, 6),
+        "plain_iterations": int(rp_last.iterations),
+        "mixed_iterations": int(rm_last.iterations),
+        "speedup_vs_plain": round(t_plain / t_mixed, 3),
+        "setup_s": {"fp64": round(t_setup_plain, 3), "fp32": round(t_setup_mixed, 3), "host_fp64": round(hs, 3),
+                    "device_fp64": round(ds, 3)},
+        "amg_levels": [[int(a), int(b)] for a, b, _ in levels],
+        "spmv_fp64_TBps": round(gbs64 / 1e3, 4),
+        "spmv_fp32_TBps": round(gbs32 / 1e3, 4),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs64, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs64 / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "spmv_stream_kernel<double,InitZero,EpiStore> (finest level)",
+            "bytes_per_launch": int(b64),
+            "us_per_launch": round(t64 * 1e6, 2),
+        },
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_info": None if cpu is None else {"cpu": cpu["cpu"], "nproc": cpu["nproc"], "setup_s": cpu["setup_s"],
+                                              "iterations": cpu["iterations"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

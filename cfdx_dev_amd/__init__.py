@@ -1,0 +1,344 @@
+"""cfdx_dev_amd: MI355X-native pressure-correction solve for CFDX.
+
+Python mirror of the reference's solver interface over the C ABI of
+libcfdxhip.so:
+
+* :class:`NativeAMGPreconditioner` follows core/linalg/hypre_amg.h:26-101
+  (setup / update_values / apply return ``bool`` and keep ``last_error``);
+* :meth:`NativeAMGPreconditioner.solve_cg` follows solve_cg_prepared
+  (core/linalg/cg_solver.h:653-667) and returns a :class:`SolverResult`
+  mirroring core/linalg/solver_status.h:45-72;
+* :meth:`NativeAMGPreconditioner.solve_fgmres` follows solve_fgmres
+  (core/linalg/gmres_solver.h:678-692).
+
+All numerical work runs in libcfdxhip.so on the GPU; nothing here computes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from ._native import AmgOptions, MeshSizes, MeshSpec, Result
+
+CONVERGED, MAX_ITER_REACHED, DIVERGED, NOT_APPLICABLE = 0, 1, 2, 3
+STATUS_NAMES = {0: "converged", 1: "max_iter_reached", 2: "diverged", 3: "not_applicable"}
+
+METHODS = {"boomer": 0, "directcf": 0, "sa": 1, "smoothed_aggregation": 1}
+POLICIES = {"low": 0, "balanced": 1, "fast": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1}
+
+
+@dataclass
+class SolverResult:
+    status: int
+    iterations: int
+    residual: float
+    residual_relative: float
+    residual_replacements: int = 0
+    solve_seconds: float = 0.0
+    preconditioner_applications: int = 0
+    extra: dict = field(default_factory=dict)
+
+    def converged(self) -> bool:
+        return self.status == CONVERGED
+
+    @staticmethod
+    def from_c(r: Result) -> "SolverResult":
+        return SolverResult(
+            status=r.status,
+            iterations=r.iterations,
+            residual=r.residual,
+            residual_relative=r.residual_relative,
+            residual_replacements=r.residual_replacements,
+            solve_seconds=r.solve_seconds,
+            preconditioner_applications=r.preconditioner_applications,
+            extra={
+                "max_true_recursive_gap": r.max_true_recursive_gap,
+                "min_true_residual": r.min_true_residual,
+                "max_true_residual": r.max_true_residual,
+                "learned_stage_applications": r.learned_stage_applications,
+                "learned_stage_fallbacks": r.learned_stage_fallbacks,
+            },
+        )
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class NativeAMGPreconditioner:
+    """GPU AMG preconditioner with the reference's lifecycle (hypre_amg.h:26-101)."""
+
+    def __init__(self, method: str = "sa", policy: str = "balanced", precision: str = "fp64",
+                 require_smoother_certificate: bool = True, reorder: bool = True,
+                 host_threads: int = 0, device: int = 0):
+        self._lib = _native.load()
+        opt = AmgOptions()
+        self._lib.cfdxhip_amg_options_default(C.byref(opt))
+        opt.method = METHODS[method]
+        opt.policy = POLICIES[policy]
+        opt.precision = PRECISIONS[precision]
+        opt.require_smoother_certificate = int(require_smoother_certificate)
+        opt.reorder = int(reorder)
+        opt.host_threads = host_threads
+        opt.device = device
+        h = C.c_void_p()
+        st = self._lib.cfdxhip_amg_create(C.byref(opt), C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"cfdxhip_amg_create failed with status {st}")
+        self._h = h
+        self.n = 0
+        self.last_status = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.cfdxhip_amg_destroy(h)
+            self._h = None
+
+    def last_error(self) -> str:
+        return self._lib.cfdxhip_amg_last_error(self._h).decode()
+
+    def _ok(self, st: int) -> bool:
+        self.last_status = st
+        return st == 0
+
+    def setup(self, n, row_offsets, columns, values) -> bool:
+        self.n = int(n)
+        return self._ok(self._lib.cfdxhip_amg_setup(self._h, self.n, _u32(row_offsets), _u32(columns),
+                                                    _f64(values)))
+
+    def update_values(self, values) -> bool:
+        return self._ok(self._lib.cfdxhip_amg_update_values(self._h, _f64(values)))
+
+    def apply(self, r):
+        z = np.empty(self.n, dtype=np.float64)
+        if not self._ok(self._lib.cfdxhip_amg_apply(self._h, _f64(r), z)):
+            return None
+        return z
+
+    def solve_cg(self, b, x, max_iter=1000, tolerance=1e-12) -> SolverResult:
+        """x is updated in place (initial guess in, solution out)."""
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        res = Result()
+        if not self._ok(self._lib.cfdxhip_amg_solve_cg(self._h, _f64(b), x, max_iter, tolerance, C.byref(res))):
+            raise RuntimeError(self.last_error())
+        return SolverResult.from_c(res)
+
+    def solve_fgmres(self, b, x, restart=30, max_iter=1000, tolerance=1e-12) -> SolverResult:
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        res = Result()
+        if not self._ok(self._lib.cfdxhip_amg_solve_fgmres(self._h, _f64(b), x, restart, max_iter, tolerance,
+                                                           C.byref(res))):
+            raise RuntimeError(self.last_error())
+        return SolverResult.from_c(res)
+
+    # ---- hierarchy introspection (reference ordering) ----
+    def num_levels(self) -> int:
+        v = C.c_size_t()
+        self._lib.cfdxhip_amg_num_levels(self._h, C.byref(v))
+        return v.value
+
+    def level_info(self, level):
+        n, nnz, pn = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        st = self._lib.cfdxhip_amg_level_info(self._h, level, C.byref(n), C.byref(nnz), C.byref(pn))
+        if st != 0:
+            raise IndexError(level)
+        return n.value, nnz.value, pn.value
+
+    def level_matrix(self, level):
+        n, nnz, _ = self.level_info(level)
+        rp = np.empty(n + 1, np.uint32)
+        cols = np.empty(nnz, np.uint32)
+        vals = np.empty(nnz, np.float64)
+        self._lib.cfdxhip_amg_level_matrix(self._h, level, rp, cols, vals)
+        return rp, cols, vals
+
+    def level_transfer(self, level):
+        n, _, pn = self.level_info(level)
+        agg = np.empty(n, np.uint64)
+        rp = np.empty(n + 1, np.uint32)
+        cols = np.empty(pn, np.uint32)
+        vals = np.empty(pn, np.float64)
+        st = self._lib.cfdxhip_amg_level_transfer(self._h, level, agg, rp, cols, vals)
+        if st != 0:
+            raise IndexError(level)
+        return agg, rp, cols, vals
+
+    def smoother_omega(self) -> float:
+        v = C.c_double()
+        self._lib.cfdxhip_amg_smoother_omega(self._h, C.byref(v))
+        return v.value
+
+    def setup_seconds(self):
+        a, b = C.c_double(), C.c_double()
+        self._lib.cfdxhip_amg_setup_seconds(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+@dataclass
+class Mesh:
+    px: np.ndarray
+    py: np.ndarray
+    pz: np.ndarray
+    face_offsets: np.ndarray
+    face_vertices: np.ndarray
+    owner: np.ndarray
+    neighbour: np.ndarray
+    cell_offsets: np.ndarray
+    cell_faces: np.ndarray
+    patch_of_face: np.ndarray
+    patch_kind: np.ndarray
+    cell_ijk: np.ndarray
+    n_cells: int
+    n_faces: int
+    n_internal_faces: int
+    n_patches: int
+
+
+MESH_KINDS = {"box": 0, "extruded2d": 1, "channel_step": 2}
+
+
+def generate_mesh(kind="box", nx=8, ny=8, nz=8, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, wall_layers=0,
+                  first_aspect=0.0, growth=0.0, step_height_fraction=0.0, step_length_fraction=0.0,
+                  permute=False, seed=0) -> Mesh:
+    """Synthetic unstructured hex mesh built by libcfdxhip's generator."""
+    lib = _native.load()
+    s = MeshSpec()
+    s.kind = MESH_KINDS[kind]
+    s.nx, s.ny, s.nz = nx, ny, nz
+    s.lx, s.ly, s.lz = lx, ly, lz
+    s.jitter = jitter
+    s.wall_layers = wall_layers
+    s.first_aspect = first_aspect
+    s.growth = growth
+    s.step_height_fraction = step_height_fraction
+    s.step_length_fraction = step_length_fraction
+    s.permute = int(permute)
+    s.seed = seed
+    h = C.c_void_p()
+    st = lib.cfdxhip_mesh_generate(C.byref(s), C.byref(h))
+    if st != 0:
+        raise ValueError(f"mesh generation failed (status {st})")
+    try:
+        z = MeshSizes()
+        lib.cfdxhip_mesh_sizes_get(h, C.byref(z))
+        a = dict(
+            px=np.empty(z.n_points, np.float64), py=np.empty(z.n_points, np.float64),
+            pz=np.empty(z.n_points, np.float64), face_offsets=np.empty(z.n_faces + 1, np.uint32),
+            face_vertices=np.empty(z.n_face_vertices, np.uint32), owner=np.empty(z.n_faces, np.uint32),
+            neighbour=np.empty(z.n_faces, np.int32), cell_offsets=np.empty(z.n_cells + 1, np.uint32),
+            cell_faces=np.empty(z.n_cell_faces, np.uint32), patch_of_face=np.empty(z.n_faces, np.uint32),
+            patch_kind=np.empty(z.n_patches, np.uint8), cell_ijk=np.empty(3 * z.n_cells, np.uint32),
+        )
+        order = ["px", "py", "pz", "face_offsets", "face_vertices", "owner", "neighbour", "cell_offsets",
+                 "cell_faces", "patch_of_face", "patch_kind", "cell_ijk"]
+        lib.cfdxhip_mesh_copy(h, *[a[k].ctypes.data_as(C.c_void_p) for k in order])
+    finally:
+        lib.cfdxhip_mesh_destroy(h)
+    return Mesh(**a, n_cells=int(z.n_cells), n_faces=int(z.n_faces), n_internal_faces=int(z.n_internal_faces),
+                n_patches=int(z.n_patches))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class FvMesh:
+    """Device-resident mesh with the pressure-path face kernels (geometry,
+    Green-Gauss gradient, mass flux, SIMPLE pressure assembly, Poisson
+    assembly). Mirrors the data the reference's corrector consumes
+    (physics/steady_incompressible_solver.h:8048-8181)."""
+
+    def __init__(self, mesh: Mesh, device: int = 0):
+        self._lib = _native.load()
+        self.mesh = mesh
+        self.keep = [np.ascontiguousarray(a) for a in (
+            mesh.px, mesh.py, mesh.pz, mesh.face_offsets, mesh.face_vertices, mesh.owner, mesh.neighbour,
+            mesh.cell_offsets, mesh.cell_faces, mesh.patch_of_face, mesh.patch_kind)]
+        k = self.keep
+        h = C.c_void_p()
+        st = self._lib.cfdxhip_fv_create(device, len(mesh.px), _ptr(k[0]), _ptr(k[1]), _ptr(k[2]), mesh.n_faces,
+                                         _ptr(k[3]), _ptr(k[4]), _ptr(k[5]), _ptr(k[6]), mesh.n_cells, _ptr(k[7]),
+                                         _ptr(k[8]), mesh.n_patches, _ptr(k[9]), _ptr(k[10]), C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"cfdxhip_fv_create failed with status {st}")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.cfdxhip_fv_destroy(h)
+            self._h = None
+
+    def last_error(self):
+        return self._lib.cfdxhip_fv_last_error(self._h).decode()
+
+    def _check(self, st):
+        if st != 0:
+            raise RuntimeError(f"status {st}: {self.last_error()}")
+
+    def geometry(self):
+        m = self.mesh
+        fc = np.empty(3 * m.n_faces)
+        fs = np.empty(3 * m.n_faces)
+        cc = np.empty(3 * m.n_cells)
+        vol = np.empty(m.n_cells)
+        self._check(self._lib.cfdxhip_fv_geometry(self._h, _ptr(fc), _ptr(fs), _ptr(cc), _ptr(vol)))
+        return fc, fs, cc, vol
+
+    def gauss_gradient(self, bc_type, bc_value, p):
+        g = np.empty(3 * self.mesh.n_cells)
+        self._check(self._lib.cfdxhip_fv_gauss_gradient(self._h, np.ascontiguousarray(bc_type, np.uint8),
+                                                        _f64(bc_value), _f64(p), g))
+        return g
+
+    def mass_flux(self, vbc_type, vbc_value, rho, U):
+        phi = np.empty(self.mesh.n_faces)
+        self._check(self._lib.cfdxhip_fv_mass_flux(self._h, np.ascontiguousarray(vbc_type, np.uint8),
+                                                   _f64(vbc_value), rho, _f64(U), phi))
+        return phi
+
+    def pressure_pattern(self, pbc_type, reference_cell=0):
+        n, nnz = C.c_size_t(), C.c_size_t()
+        t = np.ascontiguousarray(pbc_type, np.uint8)
+        self._check(self._lib.cfdxhip_fv_pressure_pattern(self._h, t, reference_cell, C.byref(n), C.byref(nnz),
+                                                          None, None))
+        rp = np.empty(n.value + 1, np.uint32)
+        cols = np.empty(nnz.value, np.uint32)
+        self._check(self._lib.cfdxhip_fv_pressure_pattern(self._h, t, reference_cell, C.byref(n), C.byref(nnz),
+                                                          _ptr(rp), _ptr(cols)))
+        return rp, cols
+
+    def pressure_assemble(self, pbc_type, pbc_value, rho, rAU, phi, p, reference_cell=0):
+        rp, cols = self.pressure_pattern(pbc_type, reference_cell)
+        vals = np.empty(len(cols))
+        rhs = np.empty(len(rp) - 1)
+        self._check(self._lib.cfdxhip_fv_pressure_assemble(self._h, np.ascontiguousarray(pbc_type, np.uint8),
+                                                           _f64(pbc_value), rho, _f64(rAU), _f64(phi), _f64(p),
+                                                           reference_cell, vals, rhs))
+        return rp, cols, vals, rhs
+
+    def diffusion_assemble(self, gamma, source, face_value, face_is_neumann):
+        nnz = C.c_size_t()
+        self._check(self._lib.cfdxhip_fv_diffusion_assemble(self._h, gamma, None, None, None, C.byref(nnz), None,
+                                                            None, None, None))
+        n = self.mesh.n_cells
+        rp = np.empty(n + 1, np.uint32)
+        cols = np.empty(nnz.value, np.uint32)
+        vals = np.empty(nnz.value)
+        rhs = np.empty(n)
+        src = _f64(source)
+        fv = _f64(face_value)
+        fn = np.ascontiguousarray(face_is_neumann, np.uint8)
+        self._check(self._lib.cfdxhip_fv_diffusion_assemble(self._h, gamma, _ptr(src), _ptr(fv), _ptr(fn),
+                                                            C.byref(nnz), _ptr(rp), _ptr(cols), _ptr(vals),
+                                                            _ptr(rhs)))
+        return rp, cols, vals, rhs

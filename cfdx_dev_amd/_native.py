@@ -1,0 +1,160 @@
+"""ctypes binding of libcfdxhip.so (the C ABI declared in include/cfdxhip.h).
+
+The library is the product: HIP kernels for gfx950 plus the host-side setup.
+Loading fails loudly when it has not been built; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcfdxhip.so")
+
+_lib = None
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class AmgOptions(C.Structure):
+    _fields_ = [
+        ("method", C.c_uint32),
+        ("policy", C.c_uint32),
+        ("require_smoother_certificate", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("reorder", C.c_uint32),
+        ("host_threads", C.c_uint32),
+        ("device", C.c_int32),
+        ("reserved", C.c_uint32 * 9),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("status", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("iterations", C.c_uint64),
+        ("residual", C.c_double),
+        ("residual_relative", C.c_double),
+        ("residual_replacements", C.c_uint64),
+        ("max_true_recursive_gap", C.c_double),
+        ("min_true_residual", C.c_double),
+        ("max_true_residual", C.c_double),
+        ("first_residual_replacement", C.c_uint64),
+        ("last_residual_replacement", C.c_uint64),
+        ("solve_seconds", C.c_double),
+        ("preconditioner_applications", C.c_uint64),
+        ("learned_stage_applications", C.c_uint64),
+        ("learned_stage_fallbacks", C.c_uint64),
+    ]
+
+
+class MeshSpec(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("nx", C.c_uint32),
+        ("ny", C.c_uint32),
+        ("nz", C.c_uint32),
+        ("lx", C.c_double),
+        ("ly", C.c_double),
+        ("lz", C.c_double),
+        ("jitter", C.c_double),
+        ("wall_layers", C.c_uint32),
+        ("permute", C.c_uint32),
+        ("first_aspect", C.c_double),
+        ("growth", C.c_double),
+        ("step_height_fraction", C.c_double),
+        ("step_length_fraction", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class MeshSizes(C.Structure):
+    _fields_ = [
+        (name, C.c_uint64)
+        for name in (
+            "n_points",
+            "n_faces",
+            "n_internal_faces",
+            "n_cells",
+            "n_face_vertices",
+            "n_cell_faces",
+            "n_patches",
+        )
+    ]
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def load():
+    """Loads libcfdxhip.so; raises if it is missing (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libcfdxhip.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    st = C.c_int
+    vp = C.c_void_p
+    _sig(lib, "cfdxhip_version", C.c_char_p, [])
+    _sig(lib, "cfdxhip_amg_options_default", None, [C.POINTER(AmgOptions)])
+    _sig(lib, "cfdxhip_amg_create", st, [C.POINTER(AmgOptions), C.POINTER(vp)])
+    _sig(lib, "cfdxhip_amg_setup", st, [vp, C.c_size_t, u32p, u32p, f64p])
+    _sig(lib, "cfdxhip_amg_update_values", st, [vp, f64p])
+    _sig(lib, "cfdxhip_amg_apply", st, [vp, f64p, f64p])
+    _sig(lib, "cfdxhip_amg_solve_cg", st, [vp, f64p, f64p, C.c_size_t, C.c_double, C.POINTER(Result)])
+    _sig(lib, "cfdxhip_amg_solve_fgmres", st,
+         [vp, f64p, f64p, C.c_int, C.c_size_t, C.c_double, C.POINTER(Result)])
+    _sig(lib, "cfdxhip_amg_solve_cg_device", st, [vp, vp, vp, C.c_size_t, C.c_double, C.POINTER(Result)])
+    _sig(lib, "cfdxhip_amg_solve_fgmres_device", st,
+         [vp, vp, vp, C.c_int, C.c_size_t, C.c_double, C.POINTER(Result)])
+    _sig(lib, "cfdxhip_amg_last_error", C.c_char_p, [vp])
+    _sig(lib, "cfdxhip_amg_destroy", None, [vp])
+    _sig(lib, "cfdxhip_amg_num_levels", st, [vp, C.POINTER(C.c_size_t)])
+    _sig(lib, "cfdxhip_amg_level_info", st,
+         [vp, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)])
+    _sig(lib, "cfdxhip_amg_level_matrix", st, [vp, C.c_size_t, u32p, u32p, f64p])
+    _sig(lib, "cfdxhip_amg_level_transfer", st, [vp, C.c_size_t, u64p, u32p, u32p, f64p])
+    _sig(lib, "cfdxhip_amg_smoother_omega", st, [vp, C.POINTER(C.c_double)])
+    _sig(lib, "cfdxhip_amg_setup_seconds", st, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)])
+    _sig(lib, "cfdxhip_amg_spmv_device", st, [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_double)])
+    _sig(lib, "cfdxhip_mesh_generate", st, [C.POINTER(MeshSpec), C.POINTER(vp)])
+    _sig(lib, "cfdxhip_mesh_sizes_get", st, [vp, C.POINTER(MeshSizes)])
+    _sig(lib, "cfdxhip_mesh_copy", st, [vp] + [vp] * 12)
+    _sig(lib, "cfdxhip_mesh_destroy", None, [vp])
+    _sig(lib, "cfdxhip_fv_create", st, [C.c_int32, C.c_size_t, vp, vp, vp, C.c_size_t, vp, vp, vp, vp,
+                                         C.c_size_t, vp, vp, C.c_size_t, vp, vp, C.POINTER(vp)])
+    _sig(lib, "cfdxhip_fv_destroy", None, [vp])
+    _sig(lib, "cfdxhip_fv_last_error", C.c_char_p, [vp])
+    _sig(lib, "cfdxhip_fv_geometry", st, [vp, vp, vp, vp, vp])
+    _sig(lib, "cfdxhip_fv_gauss_gradient", st, [vp, u8p, f64p, f64p, f64p])
+    _sig(lib, "cfdxhip_fv_mass_flux", st, [vp, u8p, f64p, C.c_double, f64p, f64p])
+    _sig(lib, "cfdxhip_fv_pressure_pattern", st,
+         [vp, u8p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), vp, vp])
+    _sig(lib, "cfdxhip_fv_pressure_assemble", st, [vp, u8p, f64p, C.c_double, f64p, f64p, f64p, C.c_size_t,
+                                                   f64p, f64p])
+    _sig(lib, "cfdxhip_fv_pressure_assemble_device", st, [vp, u8p, f64p, C.c_double, vp, vp, vp, C.c_size_t,
+                                                          vp, vp])
+    _sig(lib, "cfdxhip_fv_diffusion_assemble", st, [vp, C.c_double, vp, vp, vp, C.POINTER(C.c_size_t), vp, vp,
+                                                    vp, vp])
+    _lib = lib
+    return lib
+
+
+def exported_symbols_declared():
+    """Names of every function declared in include/cfdxhip.h."""
+    import re
+
+    hdr = os.path.join(_HERE, "..", "include", "cfdxhip.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"\b(cfdxhip_[a-z0-9_]+)\s*\(", text)))

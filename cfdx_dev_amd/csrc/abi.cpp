@@ -1,0 +1,655 @@
+// C ABI of libcfdxhip (declared in include/cfdxhip.h). Every entry point
+// catches exceptions and reports a status; the failure text is kept on the
+// handle (NativeAMGPreconditioner::last_error, core/linalg/hypre_amg.h:89-96).
+
+#include "cfdxhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "amg_setup.h"
+#include "device_amg.hpp"
+#include "fv.hpp"
+#include "mesh_gen.h"
+
+using namespace cfdxhip;
+
+struct cfdxhip_amg {
+    cfdxhip_amg_options opt{};
+    AmgConfig cfg{};
+    HostHierarchy host;
+    std::unique_ptr<DeviceAmg> dev;
+    hipStream_t stream = nullptr;
+    std::string error;
+    double host_setup_s = 0.0, device_setup_s = 0.0;
+    bool ready = false;
+};
+
+struct cfdxhip_host_mesh {
+    HostMesh m;
+};
+
+struct cfdxhip_fv {
+    FvMesh mesh;
+    std::string error;
+};
+
+namespace {
+
+const char* kEmpty = "";
+
+cfdxhip_status fail(cfdxhip_amg* h, cfdxhip_status s, const std::string& msg) {
+    if (h) h->error = msg;
+    return s;
+}
+
+template <class F>
+cfdxhip_status guarded(cfdxhip_amg* h, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(h, CFDXHIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        const std::string w = e.what();
+        const bool dev = w.rfind("HIP error", 0) == 0;
+        return fail(h, dev ? CFDXHIP_ERR_DEVICE : CFDXHIP_ERR_INTERNAL, w);
+    } catch (...) {
+        return fail(h, CFDXHIP_ERR_INTERNAL, "unknown exception");
+    }
+}
+
+void configure(cfdxhip_amg* h) {
+    AmgConfig c;
+    c.method = h->opt.method == CFDXHIP_AMG_BOOMER_STYLE ? AMG_BOOMER_STYLE : AMG_SMOOTHED_AGGREGATION;
+    // NativeAMGPreconditioner::setup policies: hypre_amg.cpp:172-186, :208.
+    if (h->opt.policy == CFDXHIP_POLICY_LOW) {
+        c.omega = 0.65;
+        c.pre = c.post = 2;
+        c.max_levels = 15;
+    } else if (h->opt.policy == CFDXHIP_POLICY_FAST) {
+        c.omega = 0.75;
+    }
+    c.require_certificate = h->opt.require_smoother_certificate != 0;
+    c.threads = h->opt.host_threads;
+    h->cfg = c;
+}
+
+void fill_result(const SolveStats& st, cfdxhip_result* r) {
+    if (!r) return;
+    std::memset(r, 0, sizeof(*r));
+    r->status = st.status;
+    r->iterations = st.iterations;
+    r->residual = st.residual;
+    r->residual_relative = st.residual_relative;
+    r->residual_replacements = st.replacements;
+    r->max_true_recursive_gap = st.max_gap;
+    r->min_true_residual = st.min_true;
+    r->max_true_residual = st.max_true;
+    r->first_residual_replacement = st.first_replacement;
+    r->last_residual_replacement = st.last_replacement;
+    r->solve_seconds = st.seconds;
+    r->preconditioner_applications = st.pc_applications;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) {
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) throw std::runtime_error("HIP error: hipMalloc failed");
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+}
+
+template <class F>
+cfdxhip_status fv_guarded(cfdxhip_fv* fv, F&& f) {
+    try {
+        hip_check(hipSetDevice(fv->mesh.dev), "hipSetDevice");
+        const cfdxhip_status s = f();
+        if (s == CFDXHIP_OK) fv->error.clear();
+        return s;
+    } catch (const std::domain_error& e) {
+        fv->error = e.what();
+        return CFDXHIP_ERR_NUMERIC;
+    } catch (const std::invalid_argument& e) {
+        fv->error = e.what();
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    } catch (const std::bad_alloc&) {
+        fv->error = "host allocation failed";
+        return CFDXHIP_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        fv->error = e.what();
+        return std::string(e.what()).rfind("HIP error", 0) == 0 ? CFDXHIP_ERR_DEVICE : CFDXHIP_ERR_INTERNAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cfdxhip_version(void) { return "cfdxhip 0.1 (gfx950)"; }
+
+void cfdxhip_amg_options_default(cfdxhip_amg_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->method = CFDXHIP_AMG_SMOOTHED_AGGREGATION;
+    o->policy = CFDXHIP_POLICY_BALANCED;
+    o->require_smoother_certificate = 1;
+    o->precision = CFDXHIP_PRECISION_FP64;
+    o->reorder = 1;
+    o->host_threads = 0;
+    o->device = 0;
+}
+
+cfdxhip_status cfdxhip_amg_create(const cfdxhip_amg_options* options, cfdxhip_amg** out) {
+    if (!out) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* h = new (std::nothrow) cfdxhip_amg();
+    if (!h) return CFDXHIP_ERR_OUT_OF_MEMORY;
+    if (options) h->opt = *options;
+    else cfdxhip_amg_options_default(&h->opt);
+    if (h->opt.method > 1 || h->opt.policy > 2 || h->opt.precision > 1) {
+        delete h;
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    }
+    configure(h);
+    // device < 0: host-only handle (hierarchy construction and introspection,
+    // no device work), used to check the host setup without a GPU.
+    const cfdxhip_status s = h->opt.device < 0 ? CFDXHIP_OK : guarded(h, [&]() {
+        hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        hip_check(hipStreamCreate(&h->stream), "hipStreamCreate");
+        return CFDXHIP_OK;
+    });
+    if (s != CFDXHIP_OK) {
+        delete h;
+        return s;
+    }
+    *out = h;
+    return CFDXHIP_OK;
+}
+
+void cfdxhip_amg_destroy(cfdxhip_amg* h) {
+    if (!h) return;
+    try {
+        if (h->opt.device >= 0) {
+            (void)hipSetDevice(h->opt.device);
+            h->dev.reset();
+            if (h->stream) (void)hipStreamDestroy(h->stream);
+        }
+    } catch (...) {
+    }
+    delete h;
+}
+
+const char* cfdxhip_amg_last_error(const cfdxhip_amg* h) { return h ? h->error.c_str() : kEmpty; }
+
+cfdxhip_status cfdxhip_amg_setup(cfdxhip_amg* h, size_t n, const uint32_t* rowp, const uint32_t* cols,
+                                 const double* vals) {
+    if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (n == 0 || n >= UINT32_MAX || !rowp || !cols || !vals)
+        return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "AMG setup requires a non-empty CSR matrix");
+    return guarded(h, [&]() {
+        const bool host_only = h->opt.device < 0;
+        if (!host_only) hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        HCsr A;
+        A.n = A.ncols = uint32_t(n);
+        A.rowp.assign(rowp, rowp + n + 1);
+        if (A.rowp[0] != 0) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "row_offsets[0] must be 0");
+        A.cols.assign(cols, cols + A.rowp[n]);
+        A.vals.assign(vals, vals + A.rowp[n]);
+        const auto t0 = std::chrono::steady_clock::now();
+        HostHierarchy staged;
+        std::string err;
+        if (!build_hierarchy(A, h->cfg, staged, err))
+            return fail(h, CFDXHIP_ERR_SETUP_FAILED, "native AMG hierarchy construction failed: " + err);
+        const auto t1 = std::chrono::steady_clock::now();
+        std::unique_ptr<DeviceAmg> dev;
+        if (!host_only) {
+            dev = std::make_unique<DeviceAmg>();
+            const bool want32 = h->opt.precision == CFDXHIP_PRECISION_FP32;
+            if (!dev->build(staged, h->cfg, h->opt.reorder != 0, true, want32, h->stream, err))
+                return fail(h, CFDXHIP_ERR_SETUP_FAILED, err);
+        }
+        const auto t2 = std::chrono::steady_clock::now();
+        // Transactional commit: the previous hierarchy survives any failure above.
+        h->host = std::move(staged);
+        h->dev = std::move(dev);
+        h->host_setup_s = std::chrono::duration<double>(t1 - t0).count();
+        h->device_setup_s = std::chrono::duration<double>(t2 - t1).count();
+        h->ready = true;
+        h->error.clear();
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_amg_update_values(cfdxhip_amg* h, const double* vals) {
+    if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready || !vals)
+        return fail(h, CFDXHIP_ERR_NOT_READY, "native AMG numeric update requires an existing compatible hierarchy");
+    return guarded(h, [&]() {
+        if (h->dev) hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        HostHierarchy staged = h->host;
+        std::string err;
+        if (!refresh_hierarchy(staged, h->cfg, vals, err)) {
+            const bool pattern = err.find("pattern") != std::string::npos;
+            return fail(h, pattern ? CFDXHIP_ERR_PATTERN_CHANGED : CFDXHIP_ERR_NUMERIC,
+                        "native AMG numeric update failed: " + err);
+        }
+        if (h->dev && !h->dev->refresh(staged, err)) return fail(h, CFDXHIP_ERR_DEVICE, err);
+        h->host = std::move(staged);
+        h->error.clear();
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_amg_apply(cfdxhip_amg* h, const double* r, double* z) {
+    if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready) return fail(h, CFDXHIP_ERR_NOT_READY, "native AMG preconditioner has not been set up");
+    if (!h->dev) return fail(h, CFDXHIP_ERR_NOT_READY, "host-only handle has no device hierarchy");
+    if (!r || !z) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "null vector");
+    return guarded(h, [&]() {
+        hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        const uint32_t n = h->dev->n();
+        for (uint32_t i = 0; i < n; ++i)
+            if (!std::isfinite(r[i])) return fail(h, CFDXHIP_ERR_NUMERIC, "non-finite residual");
+        DevBuf dr(size_t(n) * 8), dz(size_t(n) * 8), ds(size_t(n) * 8), dt(size_t(n) * 8);
+        hip_check(hipMemcpy(dr.p, r, size_t(n) * 8, hipMemcpyHostToDevice), "copy r");
+        const std::vector<uint32_t>& perm = h->dev->perm0();
+        DevBuf dperm(size_t(n) * 4);
+        hip_check(hipMemcpy(dperm.p, perm.data(), size_t(n) * 4, hipMemcpyHostToDevice), "copy perm");
+        bool ok = false;
+        if (h->opt.precision == CFDXHIP_PRECISION_FP32) {
+            DevBuf s32(size_t(n) * 4), z32(size_t(n) * 4);
+            scatter_d2f(n, dperm.as<uint32_t>(), dr.as<double>(), s32.as<float>(), h->dev->stream());
+            ok = h->dev->vcycle32(s32.as<float>(), z32.as<float>());
+            if (ok) gather_f2d(n, dperm.as<uint32_t>(), z32.as<float>(), dz.as<double>(), h->dev->stream());
+        } else {
+            scatter_d2d(n, dperm.as<uint32_t>(), dr.as<double>(), ds.as<double>(), h->dev->stream());
+            ok = h->dev->vcycle64(ds.as<double>(), dt.as<double>());
+            if (ok) gather_d2d(n, dperm.as<uint32_t>(), dt.as<double>(), dz.as<double>(), h->dev->stream());
+        }
+        hip_check(hipStreamSynchronize(h->dev->stream()), "sync");
+        if (!ok) return fail(h, CFDXHIP_ERR_NUMERIC, "native AMG V-cycle failed: singular coarsest level");
+        hip_check(hipMemcpy(z, dz.p, size_t(n) * 8, hipMemcpyDeviceToHost), "copy z");
+        for (uint32_t i = 0; i < n; ++i)
+            if (!std::isfinite(z[i])) return fail(h, CFDXHIP_ERR_NUMERIC, "native AMG V-cycle failed: non-finite correction");
+        h->error.clear();
+        return CFDXHIP_OK;
+    });
+}
+
+static cfdxhip_status solve_common(cfdxhip_amg* h, const double* b, double* x, bool device, bool fgmres,
+                                   int restart, size_t max_iter, double tol, cfdxhip_result* result) {
+    if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready) return fail(h, CFDXHIP_ERR_NOT_READY, "native AMG preconditioner has not been set up");
+    if (!h->dev) return fail(h, CFDXHIP_ERR_NOT_READY, "host-only handle has no device hierarchy");
+    if (!b || !x || max_iter == 0 || !(tol > 0.0) || !std::isfinite(tol))
+        return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "invalid solve arguments");
+    return guarded(h, [&]() {
+        hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        const uint32_t n = h->dev->n();
+        SolveStats st;
+        std::string err;
+        bool ok;
+        const bool mixed = h->opt.precision == CFDXHIP_PRECISION_FP32;
+        if (device) {
+            ok = fgmres ? h->dev->fgmres(b, x, restart, max_iter, tol, mixed, st, err)
+                        : h->dev->pcg(b, x, max_iter, tol, st, err);
+        } else {
+            DevBuf db(size_t(n) * 8), dx(size_t(n) * 8);
+            hip_check(hipMemcpy(db.p, b, size_t(n) * 8, hipMemcpyHostToDevice), "copy b");
+            hip_check(hipMemcpy(dx.p, x, size_t(n) * 8, hipMemcpyHostToDevice), "copy x");
+            ok = fgmres ? h->dev->fgmres(db.as<double>(), dx.as<double>(), restart, max_iter, tol, mixed, st, err)
+                        : h->dev->pcg(db.as<double>(), dx.as<double>(), max_iter, tol, st, err);
+            hip_check(hipMemcpy(x, dx.p, size_t(n) * 8, hipMemcpyDeviceToHost), "copy x");
+        }
+        if (!ok) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, err);
+        fill_result(st, result);
+        h->error.clear();
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_amg_solve_cg(cfdxhip_amg* h, const double* b, double* x, size_t max_iter, double tol,
+                                    cfdxhip_result* result) {
+    return solve_common(h, b, x, false, false, 0, max_iter, tol, result);
+}
+cfdxhip_status cfdxhip_amg_solve_fgmres(cfdxhip_amg* h, const double* b, double* x, int restart, size_t max_iter,
+                                        double tol, cfdxhip_result* result) {
+    if (h && restart <= 0) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "restart must be positive");
+    return solve_common(h, b, x, false, true, restart, max_iter, tol, result);
+}
+cfdxhip_status cfdxhip_amg_solve_cg_device(cfdxhip_amg* h, const double* b, double* x, size_t max_iter,
+                                           double tol, cfdxhip_result* result) {
+    return solve_common(h, b, x, true, false, 0, max_iter, tol, result);
+}
+cfdxhip_status cfdxhip_amg_solve_fgmres_device(cfdxhip_amg* h, const double* b, double* x, int restart,
+                                               size_t max_iter, double tol, cfdxhip_result* result) {
+    if (h && restart <= 0) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "restart must be positive");
+    return solve_common(h, b, x, true, true, restart, max_iter, tol, result);
+}
+
+cfdxhip_status cfdxhip_amg_num_levels(const cfdxhip_amg* h, size_t* levels) {
+    if (!h || !levels) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready) return CFDXHIP_ERR_NOT_READY;
+    *levels = h->host.lv.size();
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_level_info(const cfdxhip_amg* h, size_t level, size_t* n, size_t* nnz, size_t* pnnz) {
+    if (!h || !n || !nnz || !pnnz) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready) return CFDXHIP_ERR_NOT_READY;
+    if (level >= h->host.lv.size()) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    const HostLevel& L = h->host.lv[level];
+    *n = L.A.n;
+    *nnz = L.A.nnz();
+    *pnnz = L.P.nnz();
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_level_matrix(const cfdxhip_amg* h, size_t level, uint32_t* rowp, uint32_t* cols,
+                                        double* vals) {
+    if (!h || !rowp || !cols || !vals) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready) return CFDXHIP_ERR_NOT_READY;
+    if (level >= h->host.lv.size()) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    const HCsr& A = h->host.lv[level].A;
+    std::memcpy(rowp, A.rowp.data(), A.rowp.size() * 4);
+    std::memcpy(cols, A.cols.data(), A.cols.size() * 4);
+    std::memcpy(vals, A.vals.data(), A.vals.size() * 8);
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_level_transfer(const cfdxhip_amg* h, size_t level, uint64_t* agg, uint32_t* prowp,
+                                          uint32_t* pcols, double* pvals) {
+    if (!h || !agg || !prowp || !pcols || !pvals) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready) return CFDXHIP_ERR_NOT_READY;
+    if (level + 1 >= h->host.lv.size()) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    const HostLevel& L = h->host.lv[level];
+    for (size_t i = 0; i < L.aggregate.size(); ++i) agg[i] = L.aggregate[i];
+    std::memcpy(prowp, L.P.rowp.data(), L.P.rowp.size() * 4);
+    std::memcpy(pcols, L.P.cols.data(), L.P.cols.size() * 4);
+    std::memcpy(pvals, L.P.vals.data(), L.P.vals.size() * 8);
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_smoother_omega(const cfdxhip_amg* h, double* omega) {
+    if (!h || !omega) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready) return CFDXHIP_ERR_NOT_READY;
+    *omega = h->host.omega;
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* hs, double* ds) {
+    if (!h || !hs || !ds) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    *hs = h->host_setup_s;
+    *ds = h->device_setup_s;
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_spmv_device(cfdxhip_amg* h, uint32_t precision, const void* x, void* y, uint32_t repeat,
+                                       double* seconds) {
+    if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready || !h->dev) return fail(h, CFDXHIP_ERR_NOT_READY, "not set up on a device");
+    if (!x || !y || precision > 1) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "invalid SpMV arguments");
+    return guarded(h, [&]() {
+        hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        double sec = 0.0;
+        std::string err;
+        if (!h->dev->spmv_caller(precision, x, y, repeat, sec, err)) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, err);
+        if (seconds) *seconds = sec;
+        return CFDXHIP_OK;
+    });
+}
+
+// ------------------------------------------------------------------ mesh --
+
+cfdxhip_status cfdxhip_mesh_generate(const cfdxhip_mesh_spec* spec, cfdxhip_host_mesh** out) {
+    if (!spec || !out) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    try {
+        MeshSpec s;
+        s.kind = spec->kind;
+        s.nx = spec->nx;
+        s.ny = spec->ny;
+        s.nz = spec->nz;
+        s.lx = spec->lx;
+        s.ly = spec->ly;
+        s.lz = spec->lz;
+        s.jitter = spec->jitter;
+        s.wall_layers = spec->wall_layers;
+        s.first_aspect = spec->first_aspect;
+        s.growth = spec->growth;
+        s.step_height_fraction = spec->step_height_fraction;
+        s.step_length_fraction = spec->step_length_fraction;
+        s.permute = spec->permute;
+        s.seed = spec->seed;
+        if (s.kind > 2 || s.jitter < 0.0 || s.jitter >= 0.5) return CFDXHIP_ERR_INVALID_ARGUMENT;
+        auto* m = new cfdxhip_host_mesh();
+        m->m = generate_mesh(s);
+        *out = m;
+        return CFDXHIP_OK;
+    } catch (const std::bad_alloc&) {
+        return CFDXHIP_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    }
+}
+
+cfdxhip_status cfdxhip_mesh_sizes_get(const cfdxhip_host_mesh* mm, cfdxhip_mesh_sizes* s) {
+    if (!mm || !s) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    const HostMesh& m = mm->m;
+    s->n_points = m.px.size();
+    s->n_faces = m.owner.size();
+    s->n_internal_faces = m.n_internal_faces;
+    s->n_cells = m.n_cells;
+    s->n_face_vertices = m.fvtx.size();
+    s->n_cell_faces = m.cfaces.size();
+    s->n_patches = m.n_patches;
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_mesh_copy(const cfdxhip_host_mesh* mm, double* px, double* py, double* pz, uint32_t* foff,
+                                 uint32_t* fvtx, uint32_t* owner, int32_t* neigh, uint32_t* coff, uint32_t* cfaces,
+                                 uint32_t* patch_of_face, uint8_t* patch_kind, uint32_t* cell_ijk) {
+    if (!mm) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    const HostMesh& m = mm->m;
+    auto cp = [](void* dst, const auto& v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(px, m.px);
+    cp(py, m.py);
+    cp(pz, m.pz);
+    cp(foff, m.foff);
+    cp(fvtx, m.fvtx);
+    cp(owner, m.owner);
+    cp(neigh, m.neigh);
+    cp(coff, m.coff);
+    cp(cfaces, m.cfaces);
+    cp(patch_of_face, m.patch_of_face);
+    cp(patch_kind, m.patch_kind);
+    cp(cell_ijk, m.cell_ijk);
+    return CFDXHIP_OK;
+}
+
+void cfdxhip_mesh_destroy(cfdxhip_host_mesh* m) { delete m; }
+
+// ----------------------------------------------------------------- fv --
+
+cfdxhip_status cfdxhip_fv_create(int32_t device, size_t np, const double* px, const double* py, const double* pz,
+                                 size_t nf, const uint32_t* foff, const uint32_t* fvtx, const uint32_t* owner,
+                                 const int32_t* neigh, size_t nc, const uint32_t* coff, const uint32_t* cf,
+                                 size_t npatch, const uint32_t* patch, const uint8_t* kind, cfdxhip_fv** out) {
+    if (!out || !px || !py || !pz || !foff || !fvtx || !owner || !neigh || !coff || !cf || !patch ||
+        (npatch && !kind) || nf == 0 || nc == 0 || device < 0)
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* fv = new (std::nothrow) cfdxhip_fv();
+    if (!fv) return CFDXHIP_ERR_OUT_OF_MEMORY;
+    fv->mesh.dev = device;
+    const cfdxhip_status s = fv_guarded(fv, [&]() {
+        fv->mesh.create(device, np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cf, npatch, patch, kind);
+        return CFDXHIP_OK;
+    });
+    if (s != CFDXHIP_OK) {
+        delete fv;
+        return s;
+    }
+    *out = fv;
+    return CFDXHIP_OK;
+}
+
+void cfdxhip_fv_destroy(cfdxhip_fv* fv) {
+    if (!fv) return;
+    (void)hipSetDevice(fv->mesh.dev);
+    if (fv->mesh.stream) (void)hipStreamDestroy(fv->mesh.stream);
+    fv->mesh.stream = nullptr;
+    delete fv;
+}
+
+const char* cfdxhip_fv_last_error(const cfdxhip_fv* fv) { return fv ? fv->error.c_str() : kEmpty; }
+
+cfdxhip_status cfdxhip_fv_geometry(cfdxhip_fv* fv, double* fc, double* fs, double* cc, double* vol) {
+    if (!fv) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        fv->mesh.geometry_out(fc, fs, cc, vol);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_gauss_gradient(cfdxhip_fv* fv, const uint8_t* bct, const double* bcv, const double* p,
+                                         double* grad) {
+    if (!fv || !p || !grad || (fv->mesh.npatch && (!bct || !bcv))) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const uint32_t np_ = std::max<uint32_t>(m.npatch, 1);
+        DevBuf dt(np_), dv(size_t(np_) * 8), dp(size_t(m.nc) * 8), dg(size_t(m.nc) * 24);
+        if (m.npatch) {
+            hip_check(hipMemcpy(dt.p, bct, m.npatch, hipMemcpyHostToDevice), "copy");
+            hip_check(hipMemcpy(dv.p, bcv, size_t(m.npatch) * 8, hipMemcpyHostToDevice), "copy");
+        }
+        hip_check(hipMemcpy(dp.p, p, size_t(m.nc) * 8, hipMemcpyHostToDevice), "copy");
+        m.gauss_gradient_device(dt.as<uint8_t>(), dv.as<double>(), dp.as<double>(), dg.as<double>());
+        hip_check(hipStreamSynchronize(m.stream), "sync");
+        hip_check(hipMemcpy(grad, dg.p, size_t(m.nc) * 24, hipMemcpyDeviceToHost), "copy");
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_mass_flux(cfdxhip_fv* fv, const uint8_t* vbt, const double* vbv, double rho,
+                                    const double* U, double* phi) {
+    if (!fv || !U || !phi || (fv->mesh.npatch && (!vbt || !vbv))) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const uint32_t np_ = std::max<uint32_t>(m.npatch, 1);
+        DevBuf dt(np_), dv(size_t(np_) * 24), du(size_t(m.nc) * 24), dphi(size_t(m.nf) * 8);
+        if (m.npatch) {
+            hip_check(hipMemcpy(dt.p, vbt, m.npatch, hipMemcpyHostToDevice), "copy");
+            hip_check(hipMemcpy(dv.p, vbv, size_t(m.npatch) * 24, hipMemcpyHostToDevice), "copy");
+        }
+        hip_check(hipMemcpy(du.p, U, size_t(m.nc) * 24, hipMemcpyHostToDevice), "copy");
+        m.mass_flux_device(dt.as<uint8_t>(), dv.as<double>(), rho, du.as<double>(), dphi.as<double>());
+        hip_check(hipStreamSynchronize(m.stream), "sync");
+        hip_check(hipMemcpy(phi, dphi.p, size_t(m.nf) * 8, hipMemcpyDeviceToHost), "copy");
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_pressure_pattern(cfdxhip_fv* fv, const uint8_t* pbt, size_t ref, size_t* n_rows,
+                                           size_t* nnz, uint32_t* rowp, uint32_t* cols) {
+    if (!fv || !n_rows || !nnz || (fv->mesh.npatch && !pbt)) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const bool fixed = m.npatch ? m.has_fixed_pressure(pbt) : false;
+        const bool eliminate = !fixed && m.nc > 1 && ref < m.nc;
+        const PressurePlan& pl = m.pressure_plan(eliminate, uint32_t(eliminate ? ref : 0));
+        *n_rows = pl.n;
+        *nnz = pl.nnz;
+        if (rowp) std::memcpy(rowp, pl.h_rowp.data(), pl.h_rowp.size() * 4);
+        if (cols) std::memcpy(cols, pl.h_cols.data(), pl.h_cols.size() * 4);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_pressure_assemble_device(cfdxhip_fv* fv, const uint8_t* pbt, const double* pbv, double rho,
+                                                   const double* rAU, const double* phi, const double* p, size_t ref,
+                                                   double* values, double* rhs) {
+    if (!fv || !rAU || !phi || !p || !values || !rhs || (fv->mesh.npatch && (!pbt || !pbv)) || !std::isfinite(rho))
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        std::vector<uint8_t> t(std::max<uint32_t>(fv->mesh.npatch, 1), 0);
+        std::vector<double> v(t.size(), 0.0);
+        if (fv->mesh.npatch) {
+            std::memcpy(t.data(), pbt, fv->mesh.npatch);
+            std::memcpy(v.data(), pbv, size_t(fv->mesh.npatch) * 8);
+        }
+        fv->mesh.pressure_assemble_device(t.data(), v.data(), rho, rAU, phi, p, ref, values, rhs);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_pressure_assemble(cfdxhip_fv* fv, const uint8_t* pbt, const double* pbv, double rho,
+                                            const double* rAU, const double* phi, const double* p, size_t ref,
+                                            double* values, double* rhs) {
+    if (!fv || !rAU || !phi || !p || !values || !rhs) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        size_t n = 0, nnz = 0;
+        const cfdxhip_status s = cfdxhip_fv_pressure_pattern(fv, pbt, ref, &n, &nnz, nullptr, nullptr);
+        if (s != CFDXHIP_OK) return s;
+        DevBuf dr(size_t(m.nc) * 24), dphi(size_t(m.nf) * 8), dp(size_t(m.nc) * 8), dv(nnz * 8), db(n * 8);
+        hip_check(hipMemcpy(dr.p, rAU, size_t(m.nc) * 24, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(dphi.p, phi, size_t(m.nf) * 8, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(dp.p, p, size_t(m.nc) * 8, hipMemcpyHostToDevice), "copy");
+        const cfdxhip_status s2 = cfdxhip_fv_pressure_assemble_device(fv, pbt, pbv, rho, dr.as<double>(),
+                                                                      dphi.as<double>(), dp.as<double>(), ref,
+                                                                      dv.as<double>(), db.as<double>());
+        if (s2 != CFDXHIP_OK) return s2;
+        hip_check(hipMemcpy(values, dv.p, nnz * 8, hipMemcpyDeviceToHost), "copy");
+        hip_check(hipMemcpy(rhs, db.p, n * 8, hipMemcpyDeviceToHost), "copy");
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_diffusion_assemble(cfdxhip_fv* fv, double gamma, const double* source, const double* fval,
+                                             const uint8_t* fneu, size_t* nnz, uint32_t* rowp, uint32_t* cols,
+                                             double* values, double* rhs) {
+    if (!fv || !nnz || !(gamma > 0.0) || !std::isfinite(gamma)) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const PressurePlan& pl = m.pressure_plan(false, 0);
+        *nnz = pl.nnz;
+        if (!rowp && !cols && !values && !rhs) return CFDXHIP_OK;
+        if (!source || !fval || !fneu || !rowp || !cols || !values || !rhs) return CFDXHIP_ERR_INVALID_ARGUMENT;
+        DevBuf ds(size_t(m.nc) * 8), dfv(size_t(m.nf) * 8), dfn(m.nf), dv(pl.nnz * 8), db(size_t(m.nc) * 8);
+        hip_check(hipMemcpy(ds.p, source, size_t(m.nc) * 8, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(dfv.p, fval, size_t(m.nf) * 8, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(dfn.p, fneu, m.nf, hipMemcpyHostToDevice), "copy");
+        m.diffusion_assemble_device(gamma, ds.as<double>(), dfv.as<double>(), dfn.as<uint8_t>(), dv.as<double>(),
+                                    db.as<double>());
+        std::memcpy(rowp, pl.h_rowp.data(), pl.h_rowp.size() * 4);
+        std::memcpy(cols, pl.h_cols.data(), pl.h_cols.size() * 4);
+        hip_check(hipMemcpy(values, dv.p, pl.nnz * 8, hipMemcpyDeviceToHost), "copy");
+        hip_check(hipMemcpy(rhs, db.p, size_t(m.nc) * 8, hipMemcpyDeviceToHost), "copy");
+        return CFDXHIP_OK;
+    });
+}
+
+}  // extern "C"

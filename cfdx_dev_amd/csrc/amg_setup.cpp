@@ -1,0 +1,966 @@
+// Host-side AMG hierarchy construction (see amg_setup.h).
+//
+// Reference anchors (core/linalg/amg_preconditioner.h unless noted):
+//   diagonal           :1401-1421
+//   strength graph     :1466-1509
+//   direct C/F + RS    :1511-1885
+//   SA aggregation     :1887-1964
+//   P smoothing        :1968-2044
+//   Galerkin P^T A P   core/linalg/galerkin_csr_product.h:33-303
+//   damping certificate:1324-1399
+//   coarsest dense LU  :2564-2666
+// Floating-point expressions keep the reference's operation order; the file
+// is compiled without FMA contraction.
+
+#include "amg_setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <queue>
+
+namespace cfdxhip {
+
+namespace {
+
+constexpr uint32_t NONE = std::numeric_limits<uint32_t>::max();
+
+bool csr_valid(const HCsr& A) {
+    if (A.n == 0 || A.rowp.size() != size_t(A.n) + 1 || A.rowp[0] != 0 || A.rowp[A.n] != A.cols.size() ||
+        A.vals.size() != A.cols.size())
+        return false;
+    for (uint32_t i = 0; i < A.n; ++i) {
+        if (A.rowp[i] > A.rowp[i + 1]) return false;
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            if (A.cols[k] >= A.ncols || !std::isfinite(A.vals[k])) return false;
+            if (k > A.rowp[i] && A.cols[k] <= A.cols[k - 1]) return false;
+        }
+    }
+    return true;
+}
+
+bool build_diagonal(const HCsr& A, std::vector<double>& inv, uint32_t threads) {
+    inv.assign(A.n, 0.0);
+    std::vector<uint8_t> bad(threads ? threads : default_threads(), 0);
+    bool ok = true;
+    parallel_chunks(A.n, threads, [&](uint64_t b, uint64_t e, uint32_t t) {
+        for (uint64_t i = b; i < e; ++i) {
+            double d = 0.0;
+            for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k)
+                if (A.cols[k] == i) d += A.vals[k];
+            if (!std::isfinite(d) || std::abs(d) <= 1e-30) {
+                bad[t] = 1;
+                continue;
+            }
+            inv[i] = 1.0 / d;
+            if (!std::isfinite(inv[i])) bad[t] = 1;
+        }
+    });
+    for (auto v : bad) ok = ok && !v;
+    return ok;
+}
+
+// Strong connections as CSR; A's rows are sorted and unique, so each strong
+// row inherits that order (the reference sorts and de-duplicates anyway).
+void strength_graph(const HCsr& A, const std::vector<double>& inv, double theta, uint32_t threads,
+                    std::vector<uint32_t>& off, std::vector<uint32_t>& col) {
+    const uint32_t n = A.n;
+    std::vector<double> thr(n, 0.0);
+    std::vector<int8_t> mode(n, -1);  // -1 none, 0 absolute-scale, 1 opposite-sign
+    std::vector<uint32_t> cnt(size_t(n) + 1, 0);
+    const double eps = std::numeric_limits<double>::epsilon();
+    parallel_for(n, threads, [&](uint64_t i) {
+        const double di = 1.0 / inv[i];
+        double row_sum = di, opp = 0.0, absmax = 0.0;
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            if (A.cols[k] == i) continue;
+            const double v = A.vals[k];
+            row_sum += v;
+            absmax = std::max(absmax, std::abs(v));
+            if (di * v < 0.0) opp = std::max(opp, std::abs(v));
+        }
+        if (std::abs(row_sum) > std::abs(di) * 0.9) return;
+        const bool use_opp = opp > 0.0;
+        const double sc = use_opp ? opp : absmax;
+        if (!(sc > 0.0) || !std::isfinite(sc)) return;
+        mode[i] = use_opp ? 1 : 0;
+        thr[i] = theta * sc;
+        uint32_t c = 0;
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            const uint32_t j = A.cols[k];
+            const double v = A.vals[k];
+            if (j == i || j >= n) continue;
+            if (use_opp && di * v >= 0.0) continue;
+            if (std::abs(v) + eps < thr[i]) continue;
+            ++c;
+        }
+        cnt[i + 1] = c;
+    });
+    off.assign(size_t(n) + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i + 1];
+    col.assign(off[n], 0);
+    parallel_for(n, threads, [&](uint64_t i) {
+        if (mode[i] < 0) return;
+        const double di = 1.0 / inv[i];
+        uint32_t w = off[i];
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            const uint32_t j = A.cols[k];
+            const double v = A.vals[k];
+            if (j == i || j >= n) continue;
+            if (mode[i] == 1 && di * v >= 0.0) continue;
+            if (std::abs(v) + eps < thr[i]) continue;
+            col[w++] = j;
+        }
+    });
+}
+
+// CSR transpose of a pattern; rows of the result list sources ascending.
+void transpose_pattern(uint32_t n, const std::vector<uint32_t>& off, const std::vector<uint32_t>& col,
+                       std::vector<uint32_t>& toff, std::vector<uint32_t>& tcol) {
+    toff.assign(size_t(n) + 1, 0);
+    for (uint32_t j : col) ++toff[j + 1];
+    for (uint32_t i = 0; i < n; ++i) toff[i + 1] += toff[i];
+    tcol.assign(col.size(), 0);
+    std::vector<uint32_t> cur(toff.begin(), toff.end() - 1);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = off[i]; k < off[i + 1]; ++k) tcol[cur[col[k]]++] = i;
+}
+
+// Hierarchical bitset: find-first, insert and erase in O(log64 n).
+struct HBitset {
+    std::vector<std::vector<uint64_t>> lv;
+    explicit HBitset(uint64_t n) {
+        uint64_t m = n;
+        do {
+            m = (m + 63) / 64;
+            lv.emplace_back(m, 0ull);
+        } while (m > 1);
+    }
+    void set(uint64_t i) {
+        for (auto& w : lv) {
+            const uint64_t word = i >> 6;
+            const bool was = w[word] != 0;
+            w[word] |= 1ull << (i & 63);
+            if (was) break;
+            i = word;
+        }
+    }
+    void clear(uint64_t i) {
+        for (auto& w : lv) {
+            const uint64_t word = i >> 6;
+            w[word] &= ~(1ull << (i & 63));
+            if (w[word] != 0) break;
+            i = word;
+        }
+    }
+    bool empty() const { return lv.back()[0] == 0; }
+    uint64_t first() const {
+        uint64_t idx = uint64_t(__builtin_ctzll(lv.back()[0]));
+        for (size_t l = lv.size() - 1; l-- > 0;) idx = idx * 64 + uint64_t(__builtin_ctzll(lv[l][idx]));
+        return idx;
+    }
+};
+
+// Greedy aggregation on the symmetrised strength graph. The reference keeps a
+// lazy max-heap of (unassigned degree, n - index): every unassigned vertex
+// always has its current entry in the heap, so the seed it pops is the
+// unassigned vertex of maximum current degree, lowest index first. Buckets
+// of hierarchical bitsets select that same vertex directly.
+uint32_t greedy_aggregate(uint32_t n, const std::vector<uint32_t>& noff, const std::vector<uint32_t>& ncol,
+                          std::vector<uint32_t>& agg) {
+    agg.assign(n, NONE);
+    std::vector<uint32_t> deg(n);
+    uint32_t maxd = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        deg[i] = noff[i + 1] - noff[i];
+        maxd = std::max(maxd, deg[i]);
+    }
+    uint32_t nc = 0;
+    const double bucket_bytes = double(maxd + 1) * double(n) / 8.0;
+    if (bucket_bytes > 1.5e9) {
+        // Degenerate degree distribution: the lazy heap itself.
+        std::priority_queue<uint64_t> heap;
+        for (uint32_t i = 0; i < n; ++i) heap.push((uint64_t(deg[i]) << 32) | uint64_t(n - i));
+        uint64_t remaining = n;
+        auto assign = [&](uint32_t v, uint32_t id) {
+            if (agg[v] != NONE) return;
+            agg[v] = id;
+            --remaining;
+            for (uint32_t k = noff[v]; k < noff[v + 1]; ++k) {
+                const uint32_t u = ncol[k];
+                if (agg[u] == NONE) heap.push((uint64_t(--deg[u]) << 32) | uint64_t(n - u));
+            }
+        };
+        while (remaining > 0) {
+            uint32_t seed = NONE;
+            while (!heap.empty()) {
+                const uint64_t top = heap.top();
+                heap.pop();
+                const uint32_t idx = n - uint32_t(top & 0xffffffffu);
+                if (agg[idx] != NONE || deg[idx] != uint32_t(top >> 32)) continue;
+                seed = idx;
+                break;
+            }
+            if (seed == NONE) break;
+            const uint32_t id = nc++;
+            assign(seed, id);
+            for (uint32_t k = noff[seed]; k < noff[seed + 1]; ++k) assign(ncol[k], id);
+        }
+        return nc;
+    }
+    std::vector<HBitset> bucket;
+    bucket.reserve(maxd + 1);
+    for (uint32_t d = 0; d <= maxd; ++d) bucket.emplace_back(n);
+    for (uint32_t i = 0; i < n; ++i) bucket[deg[i]].set(i);
+    int64_t cur = maxd;
+    uint64_t remaining = n;
+    auto assign = [&](uint32_t v, uint32_t id) {
+        if (agg[v] != NONE) return;
+        agg[v] = id;
+        bucket[deg[v]].clear(v);
+        --remaining;
+        for (uint32_t k = noff[v]; k < noff[v + 1]; ++k) {
+            const uint32_t u = ncol[k];
+            if (agg[u] != NONE) continue;
+            bucket[deg[u]].clear(u);
+            --deg[u];
+            bucket[deg[u]].set(u);
+        }
+    };
+    while (remaining > 0) {
+        while (cur >= 0 && bucket[size_t(cur)].empty()) --cur;
+        if (cur < 0) break;
+        const uint32_t seed = uint32_t(bucket[size_t(cur)].first());
+        const uint32_t id = nc++;
+        assign(seed, id);
+        for (uint32_t k = noff[seed]; k < noff[seed + 1]; ++k) assign(ncol[k], id);
+    }
+    return nc;
+}
+
+// One damped-Jacobi smoothing step of the tentative prolongator, row
+// normalised (P * 1 = 1), with the reference's drop/fallback rules.
+void smooth_tentative(const HCsr& A, const std::vector<double>& inv, const std::vector<uint32_t>& agg,
+                      uint32_t nc, uint32_t threads, HCsr& P, uint64_t& fallback) {
+    const uint32_t n = A.n;
+    const double w = 2.0 / 3.0;
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows_by_chunk;
+    std::vector<uint32_t> cnt(size_t(n) + 1, 0);
+    std::vector<uint32_t> tmp_cols;
+    std::vector<double> tmp_vals;
+    // Per-row results are written to fixed-stride scratch (row degree bound),
+    // then compacted.
+    const uint32_t stride_max = [&]() {
+        uint32_t m = 1;
+        for (uint32_t i = 0; i < n; ++i) m = std::max(m, A.rowp[i + 1] - A.rowp[i] + 1);
+        return m;
+    }();
+    std::vector<uint32_t> scol(size_t(n) * stride_max);
+    std::vector<double> sval(size_t(n) * stride_max);
+    std::vector<uint8_t> fb(n, 0);
+    parallel_chunks(n, threads, [&](uint64_t b, uint64_t e, uint32_t) {
+        std::vector<std::pair<uint32_t, double>> wts;
+        wts.reserve(stride_max);
+        for (uint64_t i = b; i < e; ++i) {
+            wts.clear();
+            auto acc = [&](uint32_t c, double v) {
+                for (auto& p : wts)
+                    if (p.first == c) {
+                        p.second += v;
+                        return;
+                    }
+                wts.emplace_back(c, 0.0);
+                wts.back().second += v;
+            };
+            acc(agg[i], 1.0);
+            for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k)
+                acc(agg[A.cols[k]], -w * inv[i] * A.vals[k]);
+            std::sort(wts.begin(), wts.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            uint32_t* oc = &scol[i * stride_max];
+            double* ov = &sval[i * stride_max];
+            uint32_t m = 0;
+            double sum = 0.0;
+            for (const auto& p : wts)
+                if (std::isfinite(p.second) && std::abs(p.second) > 1e-14) sum += p.second;
+            if (!std::isfinite(sum) || std::abs(sum) <= 1e-14) {
+                oc[0] = agg[i];
+                ov[0] = 1.0;
+                cnt[i + 1] = 1;
+                fb[i] = 1;
+                continue;
+            }
+            double largest = 0.0;
+            for (const auto& p : wts) largest = std::max(largest, std::abs(p.second / sum));
+            for (const auto& p : wts) {
+                const double nv = p.second / sum;
+                if (std::isfinite(nv) && std::abs(nv) > 1e-14 && std::abs(nv) >= 0.0 * largest) {
+                    oc[m] = p.first;
+                    ov[m] = nv;
+                    ++m;
+                }
+            }
+            double kept = 0.0;
+            for (uint32_t q = 0; q < m; ++q) kept += ov[q];
+            if (std::isfinite(kept) && std::abs(kept) > 1e-14) {
+                for (uint32_t q = 0; q < m; ++q) ov[q] /= kept;
+            } else {
+                m = 0;
+            }
+            if (m == 0) {
+                oc[0] = agg[i];
+                ov[0] = 1.0;
+                m = 1;
+                fb[i] = 1;
+            }
+            cnt[i + 1] = m;
+        }
+    });
+    P.n = n;
+    P.ncols = nc;
+    P.rowp.assign(size_t(n) + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) P.rowp[i + 1] = P.rowp[i] + cnt[i + 1];
+    P.cols.resize(P.rowp[n]);
+    P.vals.resize(P.rowp[n]);
+    parallel_for(n, threads, [&](uint64_t i) {
+        const uint32_t m = P.rowp[i + 1] - P.rowp[i];
+        std::memcpy(&P.cols[P.rowp[i]], &scol[i * stride_max], m * sizeof(uint32_t));
+        std::memcpy(&P.vals[P.rowp[i]], &sval[i * stride_max], m * sizeof(double));
+    });
+    for (uint8_t f : fb) fallback += f;
+}
+
+bool sa_level(const HCsr& A, const std::vector<double>& inv, double theta, uint32_t threads, HCsr& P,
+              std::vector<uint32_t>& agg, uint32_t& nc, uint64_t& fallback) {
+    std::vector<uint32_t> soff, scol, toff, tcol;
+    strength_graph(A, inv, theta, threads, soff, scol);
+    transpose_pattern(A.n, soff, scol, toff, tcol);
+    // Symmetrised neighbourhood: sorted union of S_i and (S^T)_i.
+    const uint32_t n = A.n;
+    std::vector<uint32_t> cnt(size_t(n) + 1, 0);
+    auto merge_count = [&](uint64_t i) {
+        uint32_t a = soff[i], ae = soff[i + 1], b = toff[i], be = toff[i + 1], c = 0;
+        while (a < ae || b < be) {
+            if (b >= be || (a < ae && scol[a] < tcol[b])) ++a;
+            else if (a >= ae || tcol[b] < scol[a]) ++b;
+            else {
+                ++a;
+                ++b;
+            }
+            ++c;
+        }
+        cnt[i + 1] = c;
+    };
+    parallel_for(n, threads, merge_count);
+    std::vector<uint32_t> noff(size_t(n) + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) noff[i + 1] = noff[i] + cnt[i + 1];
+    std::vector<uint32_t> ncol(noff[n]);
+    parallel_for(n, threads, [&](uint64_t i) {
+        uint32_t a = soff[i], ae = soff[i + 1], b = toff[i], be = toff[i + 1], w = noff[i];
+        while (a < ae || b < be) {
+            if (b >= be || (a < ae && scol[a] < tcol[b])) ncol[w++] = scol[a++];
+            else if (a >= ae || tcol[b] < scol[a]) ncol[w++] = tcol[b++];
+            else {
+                ncol[w++] = scol[a];
+                ++a;
+                ++b;
+            }
+        }
+    });
+    nc = greedy_aggregate(n, noff, ncol, agg);
+    smooth_tentative(A, inv, agg, nc, threads, P, fallback);
+    return true;
+}
+
+double aval(const HCsr& A, uint32_t i, uint32_t j) {
+    const uint32_t* b = A.cols.data() + A.rowp[i];
+    const uint32_t* e = A.cols.data() + A.rowp[i + 1];
+    const uint32_t* it = std::lower_bound(b, e, j);
+    return (it != e && *it == j) ? A.vals[size_t(it - A.cols.data())] : 0.0;
+}
+
+// Ruge-Stueben splitting with classical interpolation, row-normalised.
+bool cf_level(const HCsr& A, const std::vector<double>& inv, double theta, uint32_t threads, HCsr& P,
+              std::vector<uint32_t>& agg, uint32_t& nc) {
+    const uint32_t n = A.n;
+    std::vector<uint32_t> soff, scol, toff, tcol;
+    strength_graph(A, inv, theta, threads, soff, scol);
+    transpose_pattern(n, soff, scol, toff, tcol);
+    enum : uint8_t { U = 0, F = 1, C = 2 };
+    std::vector<uint8_t> pt(n, U);
+    std::vector<uint32_t> infl(n);
+    std::priority_queue<uint64_t> heap;
+    for (uint32_t i = 0; i < n; ++i) {
+        infl[i] = toff[i + 1] - toff[i];
+        heap.push((uint64_t(infl[i]) << 32) | uint64_t(n - i));
+    }
+    uint64_t undecided = n;
+    while (undecided > 0) {
+        uint32_t best = NONE;
+        while (!heap.empty()) {
+            const uint64_t top = heap.top();
+            heap.pop();
+            const uint32_t i = n - uint32_t(top & 0xffffffffu);
+            if (pt[i] == U && uint32_t(top >> 32) == infl[i]) {
+                best = i;
+                break;
+            }
+        }
+        if (best == NONE) return false;
+        pt[best] = C;
+        --undecided;
+        for (uint32_t q = toff[best]; q < toff[best + 1]; ++q) {
+            const uint32_t j = tcol[q];
+            if (pt[j] != U) continue;
+            pt[j] = F;
+            --undecided;
+            for (uint32_t r = soff[j]; r < soff[j + 1]; ++r) {
+                const uint32_t k = scol[r];
+                if (pt[k] == U) heap.push((uint64_t(++infl[k]) << 32) | uint64_t(n - k));
+            }
+        }
+    }
+    auto common_c = [&](uint32_t i, uint32_t j) {
+        for (uint32_t a = soff[i]; a < soff[i + 1]; ++a) {
+            const uint32_t ci = scol[a];
+            if (pt[ci] != C) continue;
+            for (uint32_t b = soff[j]; b < soff[j + 1]; ++b)
+                if (scol[b] == ci) return true;
+        }
+        return false;
+    };
+    // Second pass: the reference rescans from index 0 after each promotion
+    // (:1617-1645); promotions only add C points, so no earlier index can
+    // acquire a violation and one ordered scan promotes the same points.
+    for (uint32_t i = 0; i < n; ++i) {
+        if (pt[i] != F) continue;
+        for (uint32_t a = soff[i]; a < soff[i + 1]; ++a) {
+            const uint32_t j = scol[a];
+            if (pt[j] != F || common_c(i, j)) continue;
+            pt[i] = C;
+            break;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (pt[i] != F) continue;
+        bool has = false;
+        for (uint32_t a = soff[i]; a < soff[i + 1] && !has; ++a) has = pt[scol[a]] == C;
+        if (!has) pt[i] = C;
+    }
+    std::vector<uint32_t> cidx(n, NONE);
+    nc = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (pt[i] == C) cidx[i] = nc++;
+    agg.assign(n, 0);
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows(n);
+    std::vector<uint8_t> failed(n, 0);
+    parallel_for(n, threads, [&](uint64_t ii) {
+        const uint32_t i = uint32_t(ii);
+        auto& row = rows[i];
+        if (pt[i] == C) {
+            row.emplace_back(cidx[i], 1.0);
+            agg[i] = cidx[i];
+            return;
+        }
+        const double diag = 1.0 / inv[i];
+        bool any_c = false;
+        for (uint32_t a = soff[i]; a < soff[i + 1]; ++a) any_c = any_c || pt[scol[a]] == C;
+        if (!any_c) {
+            failed[i] = 1;
+            return;
+        }
+        double weak = 0.0;
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            const uint32_t j = A.cols[k];
+            if (j == i) continue;
+            if (!std::binary_search(scol.begin() + soff[i], scol.begin() + soff[i + 1], j)) weak += A.vals[k];
+        }
+        const double den = diag + weak;
+        if (!std::isfinite(den) || std::abs(den) <= 1e-30) {
+            failed[i] = 1;
+            return;
+        }
+        std::vector<std::pair<uint32_t, double>> wts;
+        auto acc = [&](uint32_t c, double v) {
+            for (auto& p : wts)
+                if (p.first == c) {
+                    p.second += v;
+                    return;
+                }
+            wts.emplace_back(c, 0.0);
+            wts.back().second += v;
+        };
+        for (uint32_t a = soff[i]; a < soff[i + 1]; ++a) {
+            const uint32_t j = scol[a];
+            if (pt[j] == C) acc(cidx[j], aval(A, i, j));
+        }
+        for (uint32_t a = soff[i]; a < soff[i + 1]; ++a) {
+            const uint32_t k = scol[a];
+            if (pt[k] != F) continue;
+            double csum = 0.0;
+            for (uint32_t b = soff[k]; b < soff[k + 1]; ++b)
+                if (pt[scol[b]] == C) csum += aval(A, k, scol[b]);
+            if (!std::isfinite(csum) || std::abs(csum) <= 1e-30) {
+                failed[i] = 1;
+                return;
+            }
+            const double aik = aval(A, i, k);
+            for (uint32_t b = soff[k]; b < soff[k + 1]; ++b) {
+                const uint32_t j = scol[b];
+                if (pt[j] != C) continue;
+                const double akj = aval(A, k, j);
+                acc(cidx[j], aik * akj / csum);
+            }
+        }
+        std::sort(wts.begin(), wts.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        double largest = -1.0;
+        for (const auto& p : wts) {
+            const double wgt = -p.second / den;
+            if (!std::isfinite(wgt)) {
+                failed[i] = 1;
+                return;
+            }
+            row.emplace_back(p.first, wgt);
+            if (std::abs(wgt) > largest) {
+                largest = std::abs(wgt);
+                agg[i] = p.first;
+            }
+        }
+        if (row.empty()) {
+            failed[i] = 1;
+            return;
+        }
+        double rs = 0.0;
+        for (const auto& p : row) rs += p.second;
+        if (!std::isfinite(rs) || std::abs(rs) <= 1e-30) {
+            failed[i] = 1;
+            return;
+        }
+        for (auto& p : row) {
+            p.second /= rs;
+            if (!std::isfinite(p.second)) failed[i] = 1;
+        }
+    });
+    for (uint8_t f : failed)
+        if (f) return false;
+    P.n = n;
+    P.ncols = nc;
+    P.rowp.assign(size_t(n) + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) P.rowp[i + 1] = P.rowp[i] + uint32_t(rows[i].size());
+    P.cols.resize(P.rowp[n]);
+    P.vals.resize(P.rowp[n]);
+    for (uint32_t i = 0; i < n; ++i)
+        for (size_t q = 0; q < rows[i].size(); ++q) {
+            P.cols[P.rowp[i] + q] = rows[i][q].first;
+            P.vals[P.rowp[i] + q] = rows[i][q].second;
+        }
+    return true;
+}
+
+// Builds rows of a sparse pattern in parallel chunks: emit(i, out) appends the
+// sorted unique column list of row i.
+template <class Emit>
+void build_pattern(uint32_t nrows, uint32_t threads, std::vector<uint32_t>& off, std::vector<uint32_t>& col,
+                   Emit&& emit) {
+    const uint32_t T = threads ? threads : default_threads();
+    std::vector<std::vector<uint32_t>> local(T), lcnt(T);
+    std::vector<uint64_t> lb(T, 0), le(T, 0);
+    parallel_chunks(nrows, T, [&](uint64_t b, uint64_t e, uint32_t t) {
+        lb[t] = b;
+        le[t] = e;
+        std::vector<uint32_t> scratch;
+        for (uint64_t i = b; i < e; ++i) {
+            scratch.clear();
+            emit(uint32_t(i), scratch);
+            local[t].insert(local[t].end(), scratch.begin(), scratch.end());
+            lcnt[t].push_back(uint32_t(scratch.size()));
+        }
+    });
+    off.assign(size_t(nrows) + 1, 0);
+    for (uint32_t t = 0; t < T; ++t)
+        for (uint64_t i = lb[t]; i < le[t]; ++i) off[i + 1] = lcnt[t][i - lb[t]];
+    for (uint32_t i = 0; i < nrows; ++i) off[i + 1] += off[i];
+    col.resize(off[nrows]);
+    for (uint32_t t = 0; t < T; ++t)
+        if (le[t] > lb[t]) std::memcpy(&col[off[lb[t]]], local[t].data(), local[t].size() * sizeof(uint32_t));
+}
+
+void galerkin_symbolic(HostLevel& L, uint32_t threads) {
+    const HCsr& A = L.A;
+    const HCsr& P = L.P;
+    const uint32_t n = A.n, nc = P.ncols;
+    L.t_off.assign(size_t(nc) + 1, 0);
+    for (uint32_t c : P.cols) ++L.t_off[c + 1];
+    for (uint32_t c = 0; c < nc; ++c) L.t_off[c + 1] += L.t_off[c];
+    L.t_row.resize(P.nnz());
+    L.t_pos.resize(P.nnz());
+    {
+        std::vector<uint32_t> cur(L.t_off.begin(), L.t_off.end() - 1);
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t q = P.rowp[i]; q < P.rowp[i + 1]; ++q) {
+                const uint32_t c = P.cols[q];
+                L.t_row[cur[c]] = i;
+                L.t_pos[cur[c]] = q - P.rowp[i];
+                ++cur[c];
+            }
+    }
+    build_pattern(n, threads, L.ap_off, L.ap_col, [&](uint32_t i, std::vector<uint32_t>& out) {
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            const uint32_t j = A.cols[k];
+            for (uint32_t q = P.rowp[j]; q < P.rowp[j + 1]; ++q) out.push_back(P.cols[q]);
+        }
+        std::sort(out.begin(), out.end());
+        out.erase(std::unique(out.begin(), out.end()), out.end());
+    });
+    build_pattern(nc, threads, L.c_off, L.c_col, [&](uint32_t c, std::vector<uint32_t>& out) {
+        for (uint32_t t = L.t_off[c]; t < L.t_off[c + 1]; ++t) {
+            const uint32_t i = L.t_row[t];
+            out.insert(out.end(), L.ap_col.begin() + L.ap_off[i], L.ap_col.begin() + L.ap_off[i + 1]);
+        }
+        std::sort(out.begin(), out.end());
+        out.erase(std::unique(out.begin(), out.end()), out.end());
+    });
+}
+
+// Numeric P^T (A P) in the reference's accumulation order. Returns the
+// unpruned coarse values (one per symbolic entry).
+bool galerkin_numeric(const HostLevel& L, const HCsr& A, uint32_t threads, std::vector<double>& cv) {
+    const HCsr& P = L.P;
+    const uint32_t n = A.n, nc = P.ncols;
+    std::vector<double> apv(L.ap_col.size(), 0.0);
+    std::vector<uint8_t> bad(n > nc ? n : nc, 0);
+    parallel_for(n, threads, [&](uint64_t i) {
+        const uint32_t* rb = L.ap_col.data() + L.ap_off[i];
+        const uint32_t* re = L.ap_col.data() + L.ap_off[i + 1];
+        double* out = apv.data() + L.ap_off[i];
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            const uint32_t j = A.cols[k];
+            for (uint32_t q = P.rowp[j]; q < P.rowp[j + 1]; ++q) {
+                const size_t pos = size_t(std::lower_bound(rb, re, P.cols[q]) - rb);
+                out[pos] += A.vals[k] * P.vals[q];
+                if (!std::isfinite(out[pos])) bad[i] = 1;
+            }
+        }
+    });
+    for (uint32_t i = 0; i < n; ++i)
+        if (bad[i]) return false;
+    cv.assign(L.c_col.size(), 0.0);
+    parallel_for(nc, threads, [&](uint64_t c) {
+        const uint32_t* rb = L.c_col.data() + L.c_off[c];
+        const uint32_t* re = L.c_col.data() + L.c_off[c + 1];
+        double* out = cv.data() + L.c_off[c];
+        for (uint32_t t = L.t_off[c]; t < L.t_off[c + 1]; ++t) {
+            const uint32_t i = L.t_row[t];
+            const double w = P.vals[P.rowp[i] + L.t_pos[t]];
+            for (uint32_t p = L.ap_off[i]; p < L.ap_off[i + 1]; ++p) {
+                const size_t pos = size_t(std::lower_bound(rb, re, L.ap_col[p]) - rb);
+                out[pos] += w * apv[p];
+                if (!std::isfinite(out[pos])) bad[c] = 1;
+            }
+        }
+    });
+    for (uint32_t c = 0; c < nc; ++c)
+        if (bad[c]) return false;
+    return true;
+}
+
+bool stable_omega(const std::vector<HostLevel>& lv, double cfg, uint32_t threads, double& omega) {
+    double mb = 0.0;
+    for (const auto& L : lv) {
+        const HCsr& A = L.A;
+        std::vector<double> sd(A.n);
+        for (uint32_t i = 0; i < A.n; ++i) {
+            if (!(L.inv_diag[i] > 0.0) || !std::isfinite(L.inv_diag[i])) return false;
+            const double d = 1.0 / L.inv_diag[i];
+            if (!(d > 0.0) || !std::isfinite(d)) return false;
+            sd[i] = std::sqrt(d);
+        }
+        const uint32_t T = threads ? threads : default_threads();
+        std::vector<double> part(T, 0.0);
+        std::vector<uint8_t> bad(T, 0);
+        parallel_chunks(A.n, T, [&](uint64_t b, uint64_t e, uint32_t t) {
+            double ub = 0.0;
+            for (uint64_t i = b; i < e; ++i) {
+                double rs = 0.0;
+                for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+                    const double sc = sd[i] * sd[A.cols[k]];
+                    if (!(sc > 0.0) || !std::isfinite(sc)) bad[t] = 1;
+                    rs += std::abs(A.vals[k]) / sc;
+                }
+                if (!std::isfinite(rs)) bad[t] = 1;
+                ub = std::max(ub, rs);
+            }
+            part[t] = ub;
+        });
+        double ub = 0.0;
+        for (uint32_t t = 0; t < T; ++t) {
+            if (bad[t]) return false;
+            ub = std::max(ub, part[t]);
+        }
+        if (!std::isfinite(ub) || !(ub > 0.0)) return false;
+        mb = std::max(mb, ub);
+    }
+    omega = std::min(cfg, 1.8 / mb);
+    return std::isfinite(omega) && omega > 0.0 && omega < 2.0;
+}
+
+}  // namespace
+
+bool build_hierarchy(const HCsr& A, const AmgConfig& cfg, HostHierarchy& out, std::string& err) {
+    out = HostHierarchy{};
+    if (!csr_valid(A) || A.n != A.ncols) {
+        err = "AMG setup requires a non-empty, square, finalized CSR matrix";
+        return false;
+    }
+    HostLevel fine;
+    fine.A = A;
+    if (!build_diagonal(fine.A, fine.inv_diag, cfg.threads)) {
+        err = "invalid or zero diagonal on finest level";
+        return false;
+    }
+    out.lv.push_back(std::move(fine));
+    while (out.lv.back().A.n > 16 && out.lv.size() < cfg.max_levels) {
+        HostLevel& L = out.lv.back();
+        HCsr P;
+        std::vector<uint32_t> agg;
+        uint32_t nc = 0;
+        if (cfg.method == AMG_SMOOTHED_AGGREGATION) {
+            sa_level(L.A, L.inv_diag, cfg.theta, cfg.threads, P, agg, nc, out.fallback_rows);
+            if (out.lv.size() == 1) {
+                std::vector<uint32_t> members(nc, 0);
+                for (uint32_t a : agg) ++members[a];
+                for (uint32_t m : members) out.singleton_aggregates += m == 1;
+            }
+        } else if (!cf_level(L.A, L.inv_diag, cfg.theta, cfg.threads, P, agg, nc)) {
+            err = "Direct-CF interpolation construction failed at level " + std::to_string(out.lv.size() - 1);
+            out.lv.clear();
+            return false;
+        }
+        if (nc == 0 || nc >= L.A.n) break;
+        L.P = std::move(P);
+        L.aggregate = std::move(agg);
+        galerkin_symbolic(L, cfg.threads);
+        std::vector<double> cv;
+        if (!galerkin_numeric(L, L.A, cfg.threads, cv)) {
+            err = "invalid Galerkin coarse matrix from level " + std::to_string(out.lv.size() - 1);
+            out.lv.clear();
+            return false;
+        }
+        HostLevel next;
+        next.A.n = next.A.ncols = nc;
+        next.A.rowp.assign(size_t(nc) + 1, 0);
+        L.c_keep.clear();
+        for (uint32_t c = 0; c < nc; ++c) {
+            for (uint32_t p = L.c_off[c]; p < L.c_off[c + 1]; ++p)
+                if (std::abs(cv[p]) > 1e-30) {
+                    L.c_keep.push_back(p);
+                    next.A.cols.push_back(L.c_col[p]);
+                    next.A.vals.push_back(cv[p]);
+                }
+            next.A.rowp[c + 1] = uint32_t(next.A.cols.size());
+        }
+        if (!csr_valid(next.A) || !build_diagonal(next.A, next.inv_diag, cfg.threads)) {
+            err = "invalid Galerkin coarse matrix or zero diagonal at level " + std::to_string(out.lv.size());
+            out.lv.clear();
+            return false;
+        }
+        out.lv.push_back(std::move(next));
+    }
+    double w = 0.0;
+    if (stable_omega(out.lv, cfg.omega, cfg.threads, w)) {
+        out.omega = w;
+    } else if (cfg.require_certificate) {
+        err = "Jacobi stability certificate requires positive finite diagonals and a finite Gershgorin bound";
+        out.lv.clear();
+        return false;
+    } else {
+        out.omega = cfg.omega;
+    }
+    return true;
+}
+
+bool refresh_hierarchy(HostHierarchy& h, const AmgConfig& cfg, const double* vals, std::string& err) {
+    if (h.lv.empty()) {
+        err = "numeric update requires an existing hierarchy";
+        return false;
+    }
+    std::vector<std::vector<double>> newvals(h.lv.size()), newinv(h.lv.size());
+    newvals[0].assign(vals, vals + h.lv[0].A.nnz());
+    for (double v : newvals[0])
+        if (!std::isfinite(v)) {
+            err = "non-finite matrix value";
+            return false;
+        }
+    HCsr tmp;
+    for (size_t l = 0; l < h.lv.size(); ++l) {
+        const HostLevel& L = h.lv[l];
+        tmp.n = L.A.n;
+        tmp.ncols = L.A.ncols;
+        tmp.rowp = L.A.rowp;
+        tmp.cols = L.A.cols;
+        tmp.vals = newvals[l];
+        if (!build_diagonal(tmp, newinv[l], cfg.threads)) {
+            err = "invalid or zero diagonal at level " + std::to_string(l);
+            return false;
+        }
+        if (l + 1 == h.lv.size()) break;
+        std::vector<double> cv;
+        if (!galerkin_numeric(L, tmp, cfg.threads, cv)) {
+            err = "non-finite Galerkin product at level " + std::to_string(l);
+            return false;
+        }
+        // The pruned graph must be unchanged (galerkin_csr_product.h:184-212).
+        size_t keep = 0;
+        for (size_t p = 0; p < cv.size(); ++p) {
+            const bool kept = std::abs(cv[p]) > 1e-30;
+            const bool was = keep < L.c_keep.size() && L.c_keep[keep] == p;
+            if (kept != was) {
+                err = "Galerkin pattern changed at level " + std::to_string(l + 1);
+                return false;
+            }
+            if (kept) {
+                newvals[l + 1].push_back(cv[p]);
+                ++keep;
+            }
+        }
+    }
+    std::vector<HostLevel> probe(h.lv.size());
+    for (size_t l = 0; l < h.lv.size(); ++l) {
+        probe[l].A.n = h.lv[l].A.n;
+        probe[l].A.ncols = h.lv[l].A.ncols;
+        probe[l].A.rowp = h.lv[l].A.rowp;
+        probe[l].A.cols = h.lv[l].A.cols;
+        probe[l].A.vals = newvals[l];
+        probe[l].inv_diag = newinv[l];
+    }
+    double w = 0.0;
+    if (!stable_omega(probe, cfg.omega, cfg.threads, w)) {
+        if (cfg.require_certificate) {
+            err = "Jacobi stability certificate failed on refreshed values";
+            return false;
+        }
+        w = h.omega;
+    }
+    for (size_t l = 0; l < h.lv.size(); ++l) {
+        h.lv[l].A.vals = std::move(newvals[l]);
+        h.lv[l].inv_diag = std::move(newinv[l]);
+    }
+    h.omega = w;
+    return true;
+}
+
+bool factor_coarsest(const HCsr& A, CoarseLU& lu) {
+    const uint32_t n = A.n;
+    lu.n = n;
+    std::vector<double> m(size_t(n) * n, 0.0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) m[size_t(i) * n + A.cols[k]] += A.vals[k];
+    double ms = 0.0;
+    for (double v : m) ms = std::max(ms, std::abs(v));
+    lu.tol = 128.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, ms);
+    lu.piv.assign(n, 0);
+    lu.factor.assign(size_t(n) * n, 0.0);
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t piv = k;
+        double pa = std::abs(m[size_t(k) * n + k]);
+        for (uint32_t i = k + 1; i < n; ++i) {
+            const double c = std::abs(m[size_t(i) * n + k]);
+            if (c > pa) {
+                pa = c;
+                piv = i;
+            }
+        }
+        if (!std::isfinite(pa) || pa <= lu.tol) return false;
+        lu.piv[k] = piv;
+        if (piv != k)
+            for (uint32_t j = k; j < n; ++j) std::swap(m[size_t(k) * n + j], m[size_t(piv) * n + j]);
+        const double dg = m[size_t(k) * n + k];
+        for (uint32_t i = k + 1; i < n; ++i) {
+            const double f = m[size_t(i) * n + k] / dg;
+            if (!std::isfinite(f)) return false;
+            lu.factor[size_t(k) * n + i] = f;
+            m[size_t(i) * n + k] = 0.0;
+            for (uint32_t j = k + 1; j < n; ++j) m[size_t(i) * n + j] -= f * m[size_t(k) * n + j];
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const double dg = m[size_t(i) * n + i];
+        if (!std::isfinite(dg) || std::abs(dg) <= lu.tol) return false;
+    }
+    lu.upper = std::move(m);
+    return true;
+}
+
+std::vector<uint32_t> rcm_ordering(const HCsr& A) {
+    const uint32_t n = A.n;
+    std::vector<uint32_t> deg(n);
+    for (uint32_t i = 0; i < n; ++i) deg[i] = A.rowp[i + 1] - A.rowp[i];
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    std::vector<uint8_t> seen(n, 0);
+    std::vector<uint32_t> level(n, 0);
+    auto bfs_last = [&](uint32_t s, std::vector<uint32_t>& visit) {
+        // BFS from s over unseen vertices (marks temporarily), returns the
+        // min-degree vertex of the last level.
+        visit.clear();
+        visit.push_back(s);
+        seen[s] = 2;
+        level[s] = 0;
+        for (size_t h = 0; h < visit.size(); ++h) {
+            const uint32_t v = visit[h];
+            for (uint32_t k = A.rowp[v]; k < A.rowp[v + 1]; ++k) {
+                const uint32_t u = A.cols[k];
+                if (seen[u]) continue;
+                seen[u] = 2;
+                level[u] = level[v] + 1;
+                visit.push_back(u);
+            }
+        }
+        const uint32_t last = level[visit.back()];
+        uint32_t best = visit.back();
+        for (auto it = visit.rbegin(); it != visit.rend() && level[*it] == last; ++it)
+            if (deg[*it] < deg[best] || (deg[*it] == deg[best] && *it < best)) best = *it;
+        for (uint32_t v : visit) seen[v] = 0;
+        return std::make_pair(best, last);
+    };
+    std::vector<uint32_t> visit, nbr;
+    for (uint32_t start = 0; start < n; ++start) {
+        if (seen[start]) continue;
+        // Pseudo-peripheral vertex: repeat BFS while the eccentricity grows.
+        uint32_t s = start;
+        auto [cand, ecc] = bfs_last(s, visit);
+        for (int it = 0; it < 4; ++it) {
+            auto [c2, e2] = bfs_last(cand, visit);
+            if (e2 <= ecc) break;
+            s = cand;
+            cand = c2;
+            ecc = e2;
+        }
+        s = cand;
+        const size_t base = order.size();
+        order.push_back(s);
+        seen[s] = 1;
+        for (size_t h = base; h < order.size(); ++h) {
+            const uint32_t v = order[h];
+            nbr.clear();
+            for (uint32_t k = A.rowp[v]; k < A.rowp[v + 1]; ++k) {
+                const uint32_t u = A.cols[k];
+                if (!seen[u]) {
+                    seen[u] = 1;
+                    nbr.push_back(u);
+                }
+            }
+            std::sort(nbr.begin(), nbr.end(), [&](uint32_t a, uint32_t b) {
+                return deg[a] != deg[b] ? deg[a] < deg[b] : a < b;
+            });
+            order.insert(order.end(), nbr.begin(), nbr.end());
+        }
+        std::reverse(order.begin() + long(base), order.end());
+    }
+    std::vector<uint32_t> new_of_old(n);
+    for (uint32_t p = 0; p < n; ++p) new_of_old[order[p]] = p;
+    return new_of_old;
+}
+
+}  // namespace cfdxhip

@@ -1,0 +1,80 @@
+// Host-side construction of the AMG hierarchy in the caller's ordering.
+//
+// The hierarchy reproduces the reference MatrixFreeVcyclePreconditioner
+// (core/linalg/amg_preconditioner.h:76-243) exactly: the same strength
+// graph, the same greedy seed order, the same transfer weights and the same
+// Galerkin coarse values (summation order preserved), so aggregates, P and
+// every coarse matrix are bit-identical to the reference. Row-parallel
+// stages run on host threads; the inherently sequential greedy step uses a
+// bucket queue that selects exactly the vertex the reference's lazy heap
+// selects. The finished hierarchy is uploaded by device_hierarchy.hip.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "parallel.h"
+
+namespace cfdxhip {
+
+struct HCsr {
+    uint32_t n = 0;      // rows
+    uint32_t ncols = 0;  // columns
+    std::vector<uint32_t> rowp, cols;
+    std::vector<double> vals;
+    uint64_t nnz() const { return cols.size(); }
+};
+
+struct HostLevel {
+    HCsr A;
+    std::vector<double> inv_diag;
+    // Transfer to the next level (empty on the coarsest level).
+    std::vector<uint32_t> aggregate;  // fine point -> coarse id (diagnostic for C/F)
+    HCsr P;                           // n_fine x n_coarse, rows in reference order
+    // Galerkin symbolic plan (galerkin_csr_product.h:33-148).
+    std::vector<uint32_t> ap_off, ap_col, c_off, c_col, t_off, t_row, t_pos;
+    // Positions of the unpruned C entries that survive |c| > 1e-30.
+    std::vector<uint32_t> c_keep;
+};
+
+enum AmgMethod : uint32_t { AMG_BOOMER_STYLE = 0, AMG_SMOOTHED_AGGREGATION = 1 };
+
+struct AmgConfig {
+    uint32_t method = AMG_SMOOTHED_AGGREGATION;
+    double omega = 0.7;        // configured damping (hypre_amg.cpp:172-186)
+    uint32_t pre = 6, post = 6;
+    double theta = 0.25;
+    uint32_t max_levels = 25;
+    bool require_certificate = true;
+    uint32_t threads = 0;      // 0: hardware concurrency
+};
+
+struct HostHierarchy {
+    std::vector<HostLevel> lv;
+    double omega = 0.7;
+    uint64_t singleton_aggregates = 0;
+    uint64_t fallback_rows = 0;
+};
+
+bool build_hierarchy(const HCsr& A, const AmgConfig& cfg, HostHierarchy& out, std::string& err);
+// Numeric-only refresh with an unchanged pattern (amg_preconditioner.h:245-314).
+bool refresh_hierarchy(HostHierarchy& h, const AmgConfig& cfg, const double* vals, std::string& err);
+// Dense LU of the coarsest level replayable on the device
+// (amg_preconditioner.h:2564-2666). Returns false when the pivot test fails.
+struct CoarseLU {
+    uint32_t n = 0;
+    std::vector<uint32_t> piv;    // pivot row chosen at step k
+    std::vector<double> factor;   // factor[k*n + i]: multiplier of step k for row i (i > k)
+    std::vector<double> upper;    // row-major U after elimination
+    double tol = 0.0;
+};
+bool factor_coarsest(const HCsr& A, CoarseLU& lu);
+
+// Locality ordering of a symmetric pattern: reverse Cuthill-McKee from a
+// pseudo-peripheral vertex, per connected component. new_of_old[i] is the
+// position of row i in the solve ordering.
+std::vector<uint32_t> rcm_ordering(const HCsr& A);
+
+}  // namespace cfdxhip

@@ -1,0 +1,1000 @@
+// Device-resident AMG hierarchy, V-cycle, PCG and FGMRES.
+//
+// Reference anchors:
+//   V-cycle / smoother / coarsest   core/linalg/amg_preconditioner.h:2369-2443, :2564-2666
+//   PCG (true residual every iter.) core/linalg/cg_solver.h:45-569
+//   FGMRES(restart)                 core/linalg/gmres_solver.h:28-600
+//
+// The hierarchy is stored in a locality ("solve") ordering: reverse
+// Cuthill-McKee on level 0, and on every coarser level the order in which
+// aggregates first appear in the finer level's ordering. Rows keep the
+// reference's entry order (columns are relabelled, not re-sorted), and the
+// restriction rows list fine points in the reference's ascending order, so
+// each per-row sum is evaluated in the reference's order: the FP64 V-cycle
+// is bit-identical to the reference's apply() whatever the ordering.
+
+#include "device_amg.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+
+#include "kernels.hpp"
+
+namespace cfdxhip {
+
+#define HIP_OK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                  \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t kGsStride = 1024;  // partial stride of the Gram-Schmidt reductions
+constexpr uint32_t kScalars = 512;    // device scalar slots
+
+template <class T>
+T* dalloc(uint64_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <class T>
+T* upload(const std::vector<T>& v, hipStream_t s) {
+    (void)s;
+    T* d = dalloc<T>(v.size());
+    if (!v.empty()) HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+inline uint32_t grid_for(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
+inline uint32_t grid_stride(uint64_t n) {
+    return uint32_t(std::min<uint64_t>((n + kBlock - 1) / kBlock, 4096));
+}
+
+std::vector<uint32_t> row_blocks(const std::vector<uint32_t>& rowp) {
+    const uint32_t n = uint32_t(rowp.size() - 1);
+    std::vector<uint32_t> b{0};
+    uint32_t row = 0;
+    while (row < n) {
+        const uint32_t start = row;
+        if (rowp[row + 1] - rowp[row] > uint32_t(kStreamNnz)) {
+            ++row;
+        } else {
+            uint32_t acc = 0;
+            while (row < n && row - start < uint32_t(kMaxRowsPerBlock) &&
+                   acc + (rowp[row + 1] - rowp[row]) <= uint32_t(kStreamNnz)) {
+                acc += rowp[row + 1] - rowp[row];
+                ++row;
+            }
+        }
+        b.push_back(row);
+    }
+    return b;
+}
+
+// Relabels a CSR: row i -> rp[i], column j -> cp[j], entry order kept.
+void permute_csr(const HCsr& A, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cp,
+                 std::vector<uint32_t>& rowp, std::vector<uint32_t>& cols, std::vector<uint32_t>& src) {
+    const uint32_t n = A.n;
+    rowp.assign(size_t(n) + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) rowp[rp[i] + 1] = A.rowp[i + 1] - A.rowp[i];
+    for (uint32_t i = 0; i < n; ++i) rowp[i + 1] += rowp[i];
+    cols.resize(A.nnz());
+    src.resize(A.nnz());
+    parallel_for(n, 0, [&](uint64_t i) {
+        uint32_t w = rowp[rp[i]];
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k, ++w) {
+            cols[w] = cp[A.cols[k]];
+            src[w] = k;
+        }
+    });
+}
+
+}  // namespace
+
+void scatter_d2d(uint32_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s) {
+    scatter_perm_kernel<double, double><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst);
+}
+void gather_d2d(uint32_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s) {
+    gather_perm_kernel<double, double><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst);
+}
+void scatter_d2f(uint32_t n, const uint32_t* perm, const double* src, float* dst, hipStream_t s) {
+    scatter_perm_kernel<double, float><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst);
+}
+void gather_f2d(uint32_t n, const uint32_t* perm, const float* src, double* dst, hipStream_t s) {
+    gather_perm_kernel<float, double><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst);
+}
+
+DeviceAmg::~DeviceAmg() { release(); }
+
+void DeviceAmg::release() {
+    auto free_csr = [](DevCsr& m) {
+        dfree(m.rowp);
+        dfree(m.cols);
+        dfree(m.blocks);
+        dfree(m.v64);
+        dfree(m.v32);
+    };
+    for (auto& L : lv_) {
+        free_csr(L.A);
+        free_csr(L.P);
+        free_csr(L.R);
+        dfree(L.inv64);
+        dfree(L.inv32);
+        dfree(L.x64[0]);
+        dfree(L.x64[1]);
+        dfree(L.b64);
+        dfree(L.r64);
+        dfree(L.x32[0]);
+        dfree(L.x32[1]);
+        dfree(L.b32);
+        dfree(L.r32);
+    }
+    lv_.clear();
+    dfree(lu_.piv);
+    dfree(lu_.factor);
+    dfree(lu_.upper);
+    lu_ = DevCoarseLU{};
+    dfree(d_perm0_);
+    dfree(partials_);
+    dfree(scal_);
+    if (h_scal_) (void)hipHostFree(h_scal_);
+    h_scal_ = nullptr;
+    for (auto*& p : work_) dfree(p);
+    work_.clear();
+    for (auto*& p : work32_) dfree(p);
+    work32_.clear();
+    for (auto*& p : basis_) dfree(p);
+    basis_.clear();
+    basis_m_ = 0;
+}
+
+bool DeviceAmg::build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder, bool want64, bool want32,
+                      hipStream_t stream, std::string& err) {
+    release();
+    stream_ = stream;
+    omega_ = h.omega;
+    pre_ = cfg.pre;
+    post_ = cfg.post;
+    has64_ = want64;
+    has32_ = want32;
+    const size_t L = h.lv.size();
+    if (L == 0) {
+        err = "empty hierarchy";
+        return false;
+    }
+    lv_.resize(L);
+    // ---- orderings
+    for (size_t l = 0; l < L; ++l) {
+        const uint32_t n = h.lv[l].A.n;
+        auto& perm = lv_[l].new_of_old;
+        const bool last_dense = (l + 1 == L) && n <= 16;
+        if (!reorder || last_dense) {
+            perm.resize(n);
+            std::iota(perm.begin(), perm.end(), 0u);
+        } else if (l == 0) {
+            perm = rcm_ordering(h.lv[0].A);
+        } else {
+            const auto& agg = h.lv[l - 1].aggregate;
+            const auto& fp = lv_[l - 1].new_of_old;
+            std::vector<uint32_t> key(n, std::numeric_limits<uint32_t>::max());
+            for (uint32_t i = 0; i < agg.size(); ++i) key[agg[i]] = std::min(key[agg[i]], fp[i]);
+            std::vector<uint32_t> order(n);
+            std::iota(order.begin(), order.end(), 0u);
+            std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                return key[a] != key[b] ? key[a] < key[b] : a < b;
+            });
+            perm.resize(n);
+            for (uint32_t p = 0; p < n; ++p) perm[order[p]] = p;
+        }
+    }
+    // ---- matrices
+    std::vector<uint32_t> rowp, cols, src;
+    auto upload_csr = [&](DevCsr& D, const HCsr& H, const std::vector<uint32_t>& rp,
+                          const std::vector<uint32_t>& cp, uint32_t ncols) {
+        permute_csr(H, rp, cp, rowp, cols, src);
+        D.n = H.n;
+        D.ncols = ncols;
+        D.nnz = H.nnz();
+        D.rowp = upload(rowp, stream_);
+        D.cols = upload(cols, stream_);
+        const auto blk = row_blocks(rowp);
+        D.nblocks = uint32_t(blk.size() - 1);
+        D.blocks = upload(blk, stream_);
+        std::vector<double> v64(H.nnz());
+        for (size_t k = 0; k < v64.size(); ++k) v64[k] = H.vals[src[k]];
+        if (has64_) D.v64 = upload(v64, stream_);
+        if (has32_) {
+            std::vector<float> v32(v64.size());
+            for (size_t k = 0; k < v32.size(); ++k) v32[k] = float(v64[k]);
+            D.v32 = upload(v32, stream_);
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+    };
+    for (size_t l = 0; l < L; ++l) {
+        const HostLevel& H = h.lv[l];
+        DevLevel& D = lv_[l];
+        D.n = H.A.n;
+        upload_csr(D.A, H.A, D.new_of_old, D.new_of_old, H.A.n);
+        std::vector<double> inv(D.n);
+        for (uint32_t i = 0; i < D.n; ++i) inv[D.new_of_old[i]] = H.inv_diag[i];
+        if (has64_) {
+            D.inv64 = upload(inv, stream_);
+            D.x64[0] = dalloc<double>(D.n);
+            D.x64[1] = dalloc<double>(D.n);
+            D.b64 = dalloc<double>(D.n);
+            D.r64 = dalloc<double>(D.n);
+        }
+        if (has32_) {
+            std::vector<float> inv32(D.n);
+            for (uint32_t i = 0; i < D.n; ++i) inv32[i] = float(inv[i]);
+            D.inv32 = upload(inv32, stream_);
+            D.x32[0] = dalloc<float>(D.n);
+            D.x32[1] = dalloc<float>(D.n);
+            D.b32 = dalloc<float>(D.n);
+            D.r32 = dalloc<float>(D.n);
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+        if (l + 1 < L) {
+            const auto& cperm = lv_[l + 1].new_of_old;
+            upload_csr(D.P, H.P, D.new_of_old, cperm, H.P.ncols);
+            // R = P^T in the reference order: coarse row c lists fine rows
+            // ascending in the reference numbering.
+            HCsr R;
+            R.n = H.P.ncols;
+            R.ncols = H.P.n;
+            R.rowp.assign(size_t(R.n) + 1, 0);
+            for (uint32_t c : H.P.cols) ++R.rowp[c + 1];
+            for (uint32_t c = 0; c < R.n; ++c) R.rowp[c + 1] += R.rowp[c];
+            R.cols.resize(H.P.nnz());
+            R.vals.resize(H.P.nnz());
+            std::vector<uint32_t> cur(R.rowp.begin(), R.rowp.end() - 1);
+            for (uint32_t i = 0; i < H.P.n; ++i)
+                for (uint32_t q = H.P.rowp[i]; q < H.P.rowp[i + 1]; ++q) {
+                    const uint32_t c = H.P.cols[q];
+                    R.cols[cur[c]] = i;
+                    R.vals[cur[c]] = H.P.vals[q];
+                    ++cur[c];
+                }
+            upload_csr(D.R, R, cperm, D.new_of_old, H.P.n);
+        }
+    }
+    // ---- coarsest dense LU
+    const HostLevel& C = h.lv.back();
+    if (C.A.n <= 16) {
+        CoarseLU lu;
+        lu_.ok = factor_coarsest(C.A, lu);
+        lu_.n = C.A.n;
+        if (lu_.ok) {
+            lu_.piv = upload(lu.piv, stream_);
+            lu_.factor = upload(lu.factor, stream_);
+            lu_.upper = upload(lu.upper, stream_);
+        }
+    }
+    const uint32_t n0 = lv_[0].n;
+    d_perm0_ = upload(lv_[0].new_of_old, stream_);
+    partial_stride_ = std::max<uint64_t>(lv_[0].A.nblocks, 4096);
+    partials_ = dalloc<double>(std::max<uint64_t>(partial_stride_ * 8, uint64_t(kGsStride) * 130));
+    scal_ = dalloc<double>(kScalars);
+    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_scal_), kScalars * sizeof(double), hipHostMallocDefault));
+    (void)n0;
+    HIP_OK(hipStreamSynchronize(stream_));
+    return true;
+}
+
+bool DeviceAmg::refresh(const HostHierarchy& h, std::string& err) {
+    if (h.lv.size() != lv_.size()) {
+        err = "hierarchy depth changed";
+        return false;
+    }
+    for (size_t l = 0; l < lv_.size(); ++l) {
+        const HostLevel& H = h.lv[l];
+        DevLevel& D = lv_[l];
+        std::vector<uint32_t> rowp, cols, src;
+        permute_csr(H.A, D.new_of_old, D.new_of_old, rowp, cols, src);
+        std::vector<double> v64(H.A.nnz());
+        for (size_t k = 0; k < v64.size(); ++k) v64[k] = H.A.vals[src[k]];
+        std::vector<double> inv(D.n);
+        for (uint32_t i = 0; i < D.n; ++i) inv[D.new_of_old[i]] = H.inv_diag[i];
+        if (has64_) {
+            HIP_OK(hipMemcpy(D.A.v64, v64.data(), v64.size() * 8, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(D.inv64, inv.data(), inv.size() * 8, hipMemcpyHostToDevice));
+        }
+        if (has32_) {
+            std::vector<float> f(v64.size()), fi(inv.size());
+            for (size_t k = 0; k < f.size(); ++k) f[k] = float(v64[k]);
+            for (size_t k = 0; k < fi.size(); ++k) fi[k] = float(inv[k]);
+            HIP_OK(hipMemcpy(D.A.v32, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(D.inv32, fi.data(), fi.size() * 4, hipMemcpyHostToDevice));
+        }
+    }
+    omega_ = h.omega;
+    const HostLevel& C = h.lv.back();
+    if (C.A.n <= 16) {
+        CoarseLU lu;
+        dfree(lu_.piv);
+        dfree(lu_.factor);
+        dfree(lu_.upper);
+        lu_.ok = factor_coarsest(C.A, lu);
+        if (lu_.ok) {
+            lu_.piv = upload(lu.piv, stream_);
+            lu_.factor = upload(lu.factor, stream_);
+            lu_.upper = upload(lu.upper, stream_);
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+    return true;
+}
+
+// ------------------------------------------------------------ V-cycle --
+
+template <class T>
+static const T* vals_of(const DevCsr& m);
+template <>
+const double* vals_of<double>(const DevCsr& m) {
+    return m.v64;
+}
+template <>
+const float* vals_of<float>(const DevCsr& m) {
+    return m.v32;
+}
+template <class T>
+static T* const* x_of(DevLevel& L);
+template <>
+double* const* x_of<double>(DevLevel& L) {
+    return L.x64;
+}
+template <>
+float* const* x_of<float>(DevLevel& L) {
+    return L.x32;
+}
+template <class T>
+static T* b_of(DevLevel& L);
+template <>
+double* b_of<double>(DevLevel& L) {
+    return L.b64;
+}
+template <>
+float* b_of<float>(DevLevel& L) {
+    return L.b32;
+}
+template <class T>
+static T* r_of(DevLevel& L);
+template <>
+double* r_of<double>(DevLevel& L) {
+    return L.r64;
+}
+template <>
+float* r_of<float>(DevLevel& L) {
+    return L.r32;
+}
+template <class T>
+static const T* inv_of(const DevLevel& L);
+template <>
+const double* inv_of<double>(const DevLevel& L) {
+    return L.inv64;
+}
+template <>
+const float* inv_of<float>(const DevLevel& L) {
+    return L.inv32;
+}
+
+template <class T, class Init, class Epi>
+static void launch_spmv(const DevCsr& M, const T* x, Init init, Epi epi, hipStream_t s) {
+    if (M.nblocks == 0) return;
+    spmv_stream_kernel<T, Init, Epi><<<M.nblocks, kBlock, 0, s>>>(M.blocks, M.nblocks, M.rowp, M.cols,
+                                                                  vals_of<T>(M), x, init, epi);
+}
+
+template <class T>
+void DeviceAmg::spmv_jacobi_sweeps(DevLevel& L, const T* b, T*& x, uint32_t sweeps, bool from_zero) {
+    T* const* xs = x_of<T>(L);
+    const T om = T(omega_);
+    uint32_t s = 0;
+    if (from_zero && sweeps > 0) {
+        jacobi_first_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, b, inv_of<T>(L), om, x);
+        s = 1;
+    }
+    for (; s < sweeps; ++s) {
+        T* xn = (x == xs[0]) ? xs[1] : xs[0];
+        launch_spmv<T>(L.A, x, InitZero<T>{}, EpiJacobi<T>{b, x, inv_of<T>(L), xn, om}, stream_);
+        x = xn;
+    }
+}
+
+template <class T>
+bool DeviceAmg::vcycle_t(uint32_t l, const T* b, T*& x) {
+    DevLevel& L = lv_[l];
+    T* const* xs = x_of<T>(L);
+    x = xs[0];
+    if (l + 1 == lv_.size()) {
+        if (L.n <= 16) {
+            if (!lu_.ok) return false;
+            coarse_lu_kernel<T><<<1, 64, 0, stream_>>>(L.n, lu_.piv, lu_.factor, lu_.upper, b, x);
+            return true;
+        }
+        spmv_jacobi_sweeps<T>(L, b, x, 24, true);
+        return true;
+    }
+    if (pre_ > 0) {
+        spmv_jacobi_sweeps<T>(L, b, x, pre_, true);
+    } else {
+        fill_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, T(0), x);
+    }
+    T* r = r_of<T>(L);
+    launch_spmv<T>(L.A, x, InitZero<T>{}, EpiResidual<T, false>{b, r, nullptr}, stream_);
+    DevLevel& Cl = lv_[l + 1];
+    T* bc = b_of<T>(Cl);
+    launch_spmv<T>(L.R, r, InitZero<T>{}, EpiStore<T>{bc}, stream_);
+    T* xc = nullptr;
+    if (!vcycle_t<T>(l + 1, bc, xc)) return false;
+    launch_spmv<T>(L.P, xc, InitFrom<T>{x}, EpiAssign<T>{x}, stream_);
+    spmv_jacobi_sweeps<T>(L, b, x, post_, false);
+    return true;
+}
+
+bool DeviceAmg::vcycle64(const double* r, double* z) {
+    double* x = nullptr;
+    if (!vcycle_t<double>(0, r, x)) return false;
+    HIP_OK(hipMemcpyAsync(z, x, size_t(lv_[0].n) * 8, hipMemcpyDeviceToDevice, stream_));
+    return true;
+}
+
+bool DeviceAmg::vcycle32(const float* r, float* z) {
+    float* x = nullptr;
+    if (!vcycle_t<float>(0, r, x)) return false;
+    HIP_OK(hipMemcpyAsync(z, x, size_t(lv_[0].n) * 4, hipMemcpyDeviceToDevice, stream_));
+    return true;
+}
+
+// ------------------------------------------------------------ helpers --
+
+bool DeviceAmg::ensure_work(uint64_t) { return true; }
+
+void DeviceAmg::reduce(const double* partials, uint32_t n, uint32_t count, double* out) {
+    reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials, n, uint32_t(partial_stride_), count, out);
+}
+
+bool DeviceAmg::fetch_scalars(uint32_t count) {
+    HIP_OK(hipMemcpyAsync(h_scal_, scal_, count * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    return true;
+}
+
+static double* need_vec(std::vector<double*>& pool, size_t idx, uint32_t n) {
+    while (pool.size() <= idx) pool.push_back(dalloc<double>(n));
+    return pool[idx];
+}
+static float* need_vec32(std::vector<float*>& pool, size_t idx, uint32_t n) {
+    while (pool.size() <= idx) pool.push_back(dalloc<float>(n));
+    return pool[idx];
+}
+
+// ---------------------------------------------------------------- PCG --
+
+bool DeviceAmg::pcg(const double* b_caller, double* x_caller, uint64_t max_iter, double tol, SolveStats& st,
+                    std::string& err) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!has64_) {
+        err = "PCG requires the FP64 hierarchy";
+        return false;
+    }
+    st = SolveStats{};
+    DevLevel& L0 = lv_[0];
+    const uint32_t n = L0.n;
+    double* b = need_vec(work_, 0, n);
+    double* x = need_vec(work_, 1, n);
+    double* r = need_vec(work_, 2, n);
+    double* rv = need_vec(work_, 3, n);
+    double* z = need_vec(work_, 4, n);
+    double* p = need_vec(work_, 5, n);
+    double* Ap = need_vec(work_, 6, n);
+    const uint32_t g = grid_for(n);
+    scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, b_caller, b);
+    scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x_caller, x);
+    const uint32_t nbA = L0.A.nblocks;
+    const uint32_t gs = grid_stride(n);
+    double* P0 = partials_;
+    double* P1 = partials_ + partial_stride_;
+    double* P2 = partials_ + 2 * partial_stride_;
+    // scal_ slots: 0 |b|^2, 1 |rv|^2, 2 pAp, 3 |r|^2, 4 r.z, 5 |rv|^2 (iteration)
+    dot_partial_kernel<double><<<gs, kBlock, 0, stream_>>>(n, b, b, P0);
+    launch_spmv<double>(L0.A, x, InitZero<double>{}, EpiResidual<double, true>{b, rv, P1}, stream_);
+    reduce(P0, gs, 1, scal_ + 0);
+    reduce(P1, nbA, 1, scal_ + 1);
+    fetch_scalars(2);
+    const double bnorm = std::sqrt(h_scal_[0]);
+    const double r0 = std::sqrt(h_scal_[1]);
+    const double ref = bnorm > 0.0 ? bnorm : r0;
+    const double tol_abs = tol * ref;
+    auto finish = [&](uint32_t status, uint64_t it, double res) {
+        st.status = status;
+        st.iterations = it;
+        st.residual = res;
+        st.residual_relative = ref > 0.0 ? res / ref : 0.0;
+        gather_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x, x_caller);
+        HIP_OK(hipStreamSynchronize(stream_));
+        st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return true;
+    };
+    auto true_residual = [&]() {
+        launch_spmv<double>(L0.A, x, InitZero<double>{}, EpiResidual<double, true>{b, rv, P1}, stream_);
+        reduce(P1, nbA, 1, scal_ + 5);
+        fetch_scalars(6);
+        return std::sqrt(h_scal_[5]);
+    };
+    st.min_true = std::numeric_limits<double>::infinity();
+    if (!std::isfinite(r0) || !std::isfinite(bnorm)) return finish(2, 0, r0);
+    if (r0 <= tol_abs) return finish(0, 0, r0);
+    HIP_OK(hipMemcpyAsync(r, rv, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+    auto precondition = [&]() {
+        ++st.pc_applications;
+        return vcycle64(r, z);
+    };
+    if (!precondition()) return finish(3, 0, true_residual());
+    HIP_OK(hipMemcpyAsync(p, z, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+    dot_partial_kernel<double><<<gs, kBlock, 0, stream_>>>(n, r, z, P0);
+    reduce(P0, gs, 1, scal_ + 4);
+    fetch_scalars(5);
+    double rsold = h_scal_[4];
+    if (!std::isfinite(rsold)) return finish(2, 0, true_residual());
+    if (!(rsold > 0.0)) return finish(3, 0, true_residual());
+    double last_repl = std::numeric_limits<double>::infinity();
+    for (uint64_t it = 1; it <= max_iter; ++it) {
+        launch_spmv<double>(L0.A, p, InitZero<double>{}, EpiStoreDot<double>{Ap, p, P0}, stream_);
+        reduce(P0, nbA, 1, scal_ + 2);
+        cg_update_kernel<<<gs, kBlock, 0, stream_>>>(n, rsold, scal_ + 2, p, Ap, x, r, P2);
+        if (!precondition()) {
+            const double tr = true_residual();
+            return finish(3, it, tr);
+        }
+        dot_partial_kernel<double><<<gs, kBlock, 0, stream_>>>(n, r, z, P0);
+        launch_spmv<double>(L0.A, x, InitZero<double>{}, EpiResidual<double, true>{b, rv, P1}, stream_);
+        reduce(P2, gs, 1, scal_ + 3);
+        reduce(P0, gs, 1, scal_ + 4);
+        reduce(P1, nbA, 1, scal_ + 5);
+        fetch_scalars(6);
+        const double pAp = h_scal_[2];
+        if (!(pAp > 0.0) || !std::isfinite(pAp)) return finish(3, it, true_residual());
+        if (!std::isfinite(rsold / pAp)) return finish(2, it, true_residual());
+        const double rec2 = h_scal_[3];
+        const double rsnew = h_scal_[4];
+        const double tr = std::sqrt(h_scal_[5]);
+        if (!std::isfinite(rec2) || !std::isfinite(tr)) return finish(2, it, tr);
+        if (!std::isfinite(rsnew)) return finish(2, it, tr);
+        st.max_true = std::max(st.max_true, tr);
+        st.min_true = std::min(st.min_true, tr);
+        if (tr <= tol_abs) return finish(0, it, tr);
+        const double rec = std::sqrt(rec2);
+        const double gap = std::abs(tr - rec);
+        const double sc = std::max(tr, rec);
+        st.max_gap = std::max(st.max_gap, gap);
+        const bool progress = tr < 0.5 * last_repl;
+        const bool collapsed = !(rsnew > 0.0);
+        const bool adaptive = std::isfinite(rec) && sc > 0.0 && gap > 0.25 * sc && progress;
+        const bool repair = collapsed && progress;
+        if (collapsed && !repair) return finish(1, it, tr);
+        if (adaptive || repair) {
+            ++st.replacements;
+            if (st.first_replacement == 0) st.first_replacement = it;
+            st.last_replacement = it;
+            last_repl = tr;
+            HIP_OK(hipMemcpyAsync(r, rv, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+            if (!precondition()) return finish(3, it, tr);
+            dot_partial_kernel<double><<<gs, kBlock, 0, stream_>>>(n, r, z, P0);
+            reduce(P0, gs, 1, scal_ + 4);
+            fetch_scalars(5);
+            const double rz = h_scal_[4];
+            if (!std::isfinite(rz) || !(rz > 0.0)) return finish(3, it, tr);
+            HIP_OK(hipMemcpyAsync(p, z, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+            rsold = rz;
+            continue;
+        }
+        const double beta = rsnew / rsold;
+        cg_direction_kernel<<<g, kBlock, 0, stream_>>>(n, beta, z, p);
+        rsold = rsnew;
+    }
+    return finish(1, max_iter, true_residual());
+}
+
+// ------------------------------------------------------------- FGMRES --
+
+namespace {
+
+constexpr int kGsTile = 8;
+
+// partials[j * stride + block] = sum_i V_j[i] * w[i] for j < m, and the
+// block's w.w at index m.
+__global__ __launch_bounds__(kBlock) void gs_dot_kernel(uint32_t n, uint32_t m, const double* __restrict__ V,
+                                                       const double* __restrict__ w, double* __restrict__ partials,
+                                                       uint32_t stride) {
+    __shared__ double red[4];
+    for (uint32_t j0 = 0; j0 <= m; j0 += kGsTile) {
+        double acc[kGsTile];
+#pragma unroll
+        for (int t = 0; t < kGsTile; ++t) acc[t] = 0.0;
+        for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+            const double wi = w[i];
+#pragma unroll
+            for (int t = 0; t < kGsTile; ++t) {
+                const uint32_t j = j0 + t;
+                if (j < m) acc[t] += V[size_t(j) * n + i] * wi;
+                else if (j == m) acc[t] += wi * wi;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kGsTile; ++t) {
+            const uint32_t j = j0 + t;
+            const double s = block_sum(acc[t], red);
+            if (j <= m && threadIdx.x == 0) partials[size_t(j) * stride + blockIdx.x] = s;
+        }
+    }
+}
+
+// w <- w - sum_j h_j V_j ; then partials as gs_dot_kernel on the new w
+// (second Gram-Schmidt pass fused with the first update), or only |w|^2
+// when dots == false.
+__global__ __launch_bounds__(kBlock) void gs_update_kernel(uint32_t n, uint32_t m, const double* __restrict__ V,
+                                                          const double* __restrict__ h, double* __restrict__ w,
+                                                          double* __restrict__ partials, uint32_t stride,
+                                                          bool dots) {
+#pragma clang fp contract(off)
+    __shared__ double red[4];
+    __shared__ double hs[512];
+    for (uint32_t j = threadIdx.x; j < m; j += kBlock) hs[j] = h[j];
+    __syncthreads();
+    const uint32_t mm = dots ? m : 0;
+    for (uint32_t j0 = 0; j0 <= mm; j0 += kGsTile) {
+        double acc[kGsTile];
+#pragma unroll
+        for (int t = 0; t < kGsTile; ++t) acc[t] = 0.0;
+        for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+            double wi = w[i];
+            if (j0 == 0) {
+                for (uint32_t j = 0; j < m; ++j) wi -= hs[j] * V[size_t(j) * n + i];
+                w[i] = wi;
+            }
+#pragma unroll
+            for (int t = 0; t < kGsTile; ++t) {
+                const uint32_t j = j0 + t;
+                if (j < mm) acc[t] += V[size_t(j) * n + i] * wi;
+                else if (j == mm) acc[t] += wi * wi;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kGsTile; ++t) {
+            const uint32_t j = j0 + t;
+            const double s = block_sum(acc[t], red);
+            if (j <= mm && threadIdx.x == 0) partials[size_t(j) * stride + blockIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void scale_copy_kernel(uint32_t n, const double* __restrict__ w,
+                                                           const double* __restrict__ sc_ptr, double* __restrict__ v) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) v[i] = w[i] / *sc_ptr;
+}
+
+__global__ __launch_bounds__(kBlock) void scale_sqrt_copy_kernel(uint32_t n, const double* __restrict__ w,
+                                                                const double* __restrict__ sq_ptr,
+                                                                double* __restrict__ v) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) v[i] = w[i] / sqrt(*sq_ptr);
+}
+
+// x += sum_j y_j Z_j (y on device)
+__global__ __launch_bounds__(kBlock) void basis_axpy_kernel(uint32_t n, uint32_t m, const double* __restrict__ Z,
+                                                           const double* __restrict__ y, double* __restrict__ x) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double xi = x[i];
+    for (uint32_t j = 0; j < m; ++j) xi += y[j] * Z[size_t(j) * n + i];
+    x[i] = xi;
+}
+
+}  // namespace
+
+bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, uint64_t max_iter, double tol,
+                       bool mixed, SolveStats& st, std::string& err) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!has64_ && !mixed) {
+        err = "FP64 FGMRES preconditioner requires the FP64 hierarchy";
+        return false;
+    }
+    if (mixed && !has32_) {
+        err = "mixed-precision FGMRES requires the FP32 hierarchy";
+        return false;
+    }
+    if (!lv_[0].A.v64) {
+        err = "FGMRES requires the FP64 finest operator";
+        return false;
+    }
+    st = SolveStats{};
+    st.min_true = std::numeric_limits<double>::infinity();
+    DevLevel& L0 = lv_[0];
+    const uint32_t n = L0.n;
+    const int rmin = 10, rmax = 120;
+    int cur = std::clamp(restart, rmin, rmax);
+    if (n < uint32_t(cur)) cur = int(n);
+    // Basis storage: V (rmax+1) and Z (rmax) FP64, allocated for the largest
+    // restart this solve can reach.
+    const uint32_t mcap = uint32_t(std::min<int>(rmax, int(n)));
+    if (basis_m_ < mcap) {
+        for (auto*& p : basis_) dfree(p);
+        basis_.clear();
+        basis_.push_back(dalloc<double>(uint64_t(mcap + 1) * n));
+        basis_.push_back(dalloc<double>(uint64_t(mcap) * n));
+        basis_m_ = mcap;
+    }
+    double* V = basis_[0];
+    double* Z = basis_[1];
+    double* b = need_vec(work_, 0, n);
+    double* x = need_vec(work_, 1, n);
+    double* r = need_vec(work_, 2, n);
+    double* w = need_vec(work_, 3, n);
+    double* bestx = need_vec(work_, 4, n);
+    double* dy = need_vec(work_, 7, n);  // small device buffer for y (n >= m)
+    float* r32 = mixed ? need_vec32(work32_, 0, n) : nullptr;
+    float* z32 = mixed ? need_vec32(work32_, 1, n) : nullptr;
+    const uint32_t g = grid_for(n);
+    const uint32_t gs = std::min<uint32_t>(grid_stride(n), kGsStride);
+    const uint32_t nbA = L0.A.nblocks;
+    scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, b_caller, b);
+    scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x_caller, x);
+    double* P0 = partials_;
+    dot_partial_kernel<double><<<gs, kBlock, 0, stream_>>>(n, b, b, P0);
+    reduce(P0, gs, 1, scal_);
+    fetch_scalars(1);
+    const double bn = std::sqrt(h_scal_[0]);
+    auto finish = [&](uint32_t status, uint64_t it, double res) {
+        st.status = status;
+        st.iterations = it;
+        st.residual = res;
+        st.residual_relative = res / std::max(bn, 1e-300);
+        gather_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x, x_caller);
+        HIP_OK(hipStreamSynchronize(stream_));
+        st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return true;
+    };
+    if (!std::isfinite(bn)) return finish(2, 0, bn);
+    if (bn == 0.0) {
+        err = "FGMRES with a zero right-hand side is not supported";
+        return finish(3, 0, 0.0);
+    }
+    const double tolv = tol * bn;
+    auto true_res = [&]() {
+        launch_spmv<double>(L0.A, x, InitZero<double>{}, EpiResidual<double, true>{b, r, P0}, stream_);
+        reduce(P0, nbA, 1, scal_);
+        fetch_scalars(1);
+        return std::sqrt(h_scal_[0]);
+    };
+    auto precondition = [&](const double* v, double* z) {
+        ++st.pc_applications;
+        if (!mixed) return vcycle64(v, z);
+        convert_kernel<double, float><<<g, kBlock, 0, stream_>>>(n, v, r32);
+        if (!vcycle32(r32, z32)) return false;
+        convert_kernel<float, double><<<g, kBlock, 0, stream_>>>(n, z32, z);
+        return true;
+    };
+    double beta = true_res();
+    if (!std::isfinite(beta)) return finish(2, 0, beta);
+    if (beta <= tolv) return finish(0, 0, beta);
+    uint64_t its = 0;
+    double prev_cycle = beta, best = beta;
+    HIP_OK(hipMemcpyAsync(bestx, x, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+    std::vector<double> H, cs, sn, gv, yv;
+    while (its < max_iter) {
+        const uint32_t m = uint32_t(cur);
+        H.assign(size_t(m + 1) * m, 0.0);
+        cs.assign(m, 0.0);
+        sn.assign(m, 0.0);
+        gv.assign(m + 1, 0.0);
+        yv.assign(m, 0.0);
+        auto Hr = [&](size_t i, size_t j) -> double& { return H[i * m + j]; };
+        beta = true_res();
+        if (!std::isfinite(beta)) {
+            HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+            return finish(2, its, best);
+        }
+        if (beta <= tolv) break;
+        // v_0 = r / beta
+        h_scal_[0] = beta;
+        HIP_OK(hipMemcpyAsync(scal_ + 8, h_scal_, 8, hipMemcpyHostToDevice, stream_));
+        scale_copy_kernel<<<g, kBlock, 0, stream_>>>(n, r, scal_ + 8, V);
+        gv[0] = beta;
+        uint32_t used = 0;
+        double est = beta;
+        bool brk = false;
+        for (uint32_t j = 0; j < m && its < max_iter; ++j) {
+            double* vj = V + size_t(j) * n;
+            double* zj = Z + size_t(j) * n;
+            if (!precondition(vj, zj)) {
+                HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+                return finish(3, its, best);
+            }
+            // w = A z_j
+            launch_spmv<double>(L0.A, zj, InitZero<double>{}, EpiStore<double>{w}, stream_);
+            // first CGS pass: h1 = V^T w, |w|^2 (slots 16..)
+            gs_dot_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, w, partials_, kGsStride);
+            reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 16);
+            // w -= V h1; h2 = V^T w (slots 160..)
+            gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 16, w, partials_, kGsStride, true);
+            reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 160);
+            // w -= V h2; |w|^2 (slot 304)
+            gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 160, w, partials_, kGsStride,
+                                                        false);
+            reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, 1, scal_ + 304);
+            HIP_OK(hipMemcpyAsync(h_scal_, scal_, 305 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            const double* hh = h_scal_;
+            const double an = std::sqrt(hh[16 + j + 1]);
+            if (!std::isfinite(an)) {
+                HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+                return finish(2, its, best);
+            }
+            for (uint32_t i = 0; i <= j; ++i) Hr(i, j) = hh[16 + i] + hh[160 + i];
+            const double hn = std::sqrt(hh[304]);
+            if (!std::isfinite(hn)) {
+                HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+                return finish(2, its, best);
+            }
+            Hr(j + 1, j) = hn;
+            const double floor_ = 4096.0 * std::numeric_limits<double>::epsilon() * std::max(an, 1e-300);
+            const bool bd = hn <= floor_ && an > 0.0;
+            if (hn > 0.0) {
+                // |w| stays in slot 304 as a square; v_{j+1} = w / sqrt(slot).
+                scale_sqrt_copy_kernel<<<g, kBlock, 0, stream_>>>(n, w, scal_ + 304, V + size_t(j + 1) * n);
+            }
+            for (uint32_t i = 0; i < j; ++i) {
+                const double h0 = Hr(i, j), h1 = Hr(i + 1, j);
+                Hr(i, j) = cs[i] * h0 + sn[i] * h1;
+                Hr(i + 1, j) = -sn[i] * h0 + cs[i] * h1;
+            }
+            const double a = Hr(j, j), b2 = Hr(j + 1, j);
+            const double rho = std::hypot(a, b2);
+            if (rho > 0.0) {
+                cs[j] = a / rho;
+                sn[j] = b2 / rho;
+                Hr(j, j) = rho;
+                Hr(j + 1, j) = 0.0;
+                const double gj = gv[j], gj1 = gv[j + 1];
+                gv[j] = cs[j] * gj + sn[j] * gj1;
+                gv[j + 1] = -sn[j] * gj + cs[j] * gj1;
+            }
+            ++its;
+            ++used;
+            est = std::abs(gv[j + 1]);
+            if (bd) {
+                brk = true;
+                break;
+            }
+            if (est <= tolv) break;
+        }
+        if (used == 0) break;
+        std::fill(yv.begin(), yv.end(), 0.0);
+        for (size_t i = used; i-- > 0;) {
+            double s = gv[i];
+            for (size_t jj = i + 1; jj < used; ++jj) s -= Hr(i, jj) * yv[jj];
+            const double dg = Hr(i, i);
+            double hs = std::abs(dg);
+            for (size_t k = 0; k < used; ++k) hs = std::max(hs, std::abs(Hr(k, i)));
+            const double fl = 128.0 * std::numeric_limits<double>::epsilon() * std::max(hs, 1e-300);
+            if (std::abs(dg) <= fl) {
+                if (std::abs(s) <= fl) {
+                    yv[i] = 0.0;
+                    continue;
+                }
+                double br = true_res();
+                if (!std::isfinite(br) || best < br) {
+                    HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+                    br = best;
+                }
+                return finish(2, its, br);
+            }
+            yv[i] = s / dg;
+        }
+        HIP_OK(hipMemcpyAsync(dy, yv.data(), used * 8, hipMemcpyHostToDevice, stream_));
+        basis_axpy_kernel<<<g, kBlock, 0, stream_>>>(n, used, Z, dy, x);
+        beta = true_res();
+        if (!std::isfinite(beta)) {
+            HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+            return finish(2, its, best);
+        }
+        st.max_true = std::max(st.max_true, beta);
+        st.min_true = std::min(st.min_true, beta);
+        if (beta < best) {
+            best = beta;
+            HIP_OK(hipMemcpyAsync(bestx, x, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+        }
+        if (beta <= tolv) return finish(0, its, beta);
+        if (brk) {
+            if (best < beta) {
+                HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+                beta = best;
+            }
+            return finish(2, its, beta);
+        }
+        const double red = beta / std::max(prev_cycle, 1e-300);
+        if (red > 0.1 * 2.0)
+            cur = cur > rmax - 5 ? rmax : cur + 5;
+        else if (red < 0.1 * 0.5)
+            cur = std::max(rmin, cur - 5);
+        if (n < uint32_t(cur)) cur = int(n);
+        prev_cycle = beta;
+    }
+    if (best < beta) {
+        HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+        beta = best;
+    }
+    return finish(beta <= tolv ? 0 : 1, its, beta);
+}
+
+// ------------------------------------------------------------ SpMV API --
+
+bool DeviceAmg::spmv_caller(uint32_t precision, const void* x_caller, void* y_caller, uint32_t repeat,
+                            double& sec, std::string& err) {
+    DevLevel& L0 = lv_[0];
+    const uint32_t n = L0.n;
+    const uint32_t g = grid_for(n);
+    if (repeat == 0) repeat = 1;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    if (precision == 0) {
+        if (!has64_) {
+            err = "no FP64 hierarchy";
+            return false;
+        }
+        double* xs = need_vec(work_, 5, n);
+        double* ys = need_vec(work_, 6, n);
+        scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, (const double*)x_caller, xs);
+        HIP_OK(hipEventRecord(e0, stream_));
+        for (uint32_t k = 0; k < repeat; ++k)
+            launch_spmv<double>(L0.A, xs, InitZero<double>{}, EpiStore<double>{ys}, stream_);
+        HIP_OK(hipEventRecord(e1, stream_));
+        gather_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, ys, (double*)y_caller);
+    } else {
+        if (!has32_) {
+            err = "no FP32 hierarchy";
+            return false;
+        }
+        float* xs = need_vec32(work32_, 2, n);
+        float* ys = need_vec32(work32_, 3, n);
+        scatter_perm_kernel<float, float><<<g, kBlock, 0, stream_>>>(n, d_perm0_, (const float*)x_caller, xs);
+        HIP_OK(hipEventRecord(e0, stream_));
+        for (uint32_t k = 0; k < repeat; ++k)
+            launch_spmv<float>(L0.A, xs, InitZero<float>{}, EpiStore<float>{ys}, stream_);
+        HIP_OK(hipEventRecord(e1, stream_));
+        gather_perm_kernel<float, float><<<g, kBlock, 0, stream_>>>(n, d_perm0_, ys, (float*)y_caller);
+    }
+    HIP_OK(hipStreamSynchronize(stream_));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sec = double(ms) * 1e-3 / double(repeat);
+    return true;
+}
+
+}  // namespace cfdxhip

@@ -1,0 +1,122 @@
+// Device-resident AMG hierarchy and Krylov solvers (HIP).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "amg_setup.h"
+
+namespace cfdxhip {
+
+struct DevCsr {
+    uint32_t n = 0, ncols = 0;
+    uint64_t nnz = 0;
+    uint32_t* rowp = nullptr;
+    uint32_t* cols = nullptr;
+    uint32_t* blocks = nullptr;  // CSR-stream row blocks (nblocks + 1 boundaries)
+    uint32_t nblocks = 0;
+    double* v64 = nullptr;
+    float* v32 = nullptr;
+};
+
+struct DevLevel {
+    uint32_t n = 0;
+    DevCsr A, P, R;  // P: n x n_coarse, R = P^T: n_coarse x n
+    double* inv64 = nullptr;
+    float* inv32 = nullptr;
+    double* x64[2] = {nullptr, nullptr};
+    double* b64 = nullptr;
+    double* r64 = nullptr;
+    float* x32[2] = {nullptr, nullptr};
+    float* b32 = nullptr;
+    float* r32 = nullptr;
+    // Host mirror of the solve-ordering permutation (new index of old row).
+    std::vector<uint32_t> new_of_old;
+};
+
+struct DevCoarseLU {
+    bool ok = false;
+    uint32_t n = 0;
+    uint32_t* piv = nullptr;
+    double* factor = nullptr;
+    double* upper = nullptr;
+};
+
+struct SolveStats {
+    uint32_t status = 3;
+    uint64_t iterations = 0;
+    double residual = 0.0, residual_relative = 0.0;
+    uint64_t replacements = 0;
+    double max_gap = 0.0, min_true = 0.0, max_true = 0.0;
+    uint64_t first_replacement = 0, last_replacement = 0;
+    uint64_t pc_applications = 0;
+    double seconds = 0.0;
+};
+
+// Caller <-> solve ordering moves on the device (new_of_old on the device).
+void scatter_d2d(uint32_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s);
+void gather_d2d(uint32_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s);
+void scatter_d2f(uint32_t n, const uint32_t* perm, const double* src, float* dst, hipStream_t s);
+void gather_f2d(uint32_t n, const uint32_t* perm, const float* src, double* dst, hipStream_t s);
+
+class DeviceAmg {
+public:
+    DeviceAmg() = default;
+    ~DeviceAmg();
+    DeviceAmg(const DeviceAmg&) = delete;
+    DeviceAmg& operator=(const DeviceAmg&) = delete;
+
+    // Uploads a host hierarchy (reference ordering) into the solve ordering.
+    bool build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder, bool want64, bool want32,
+               hipStream_t stream, std::string& err);
+    // Re-uploads values after a numeric refresh (same patterns).
+    bool refresh(const HostHierarchy& h, std::string& err);
+
+    uint32_t n() const { return lv_.empty() ? 0 : lv_[0].n; }
+    // One V-cycle z = M^{-1} r. r, z: device vectors in the SOLVE ordering of
+    // level 0, precision T. Returns false if the coarse solve is unusable.
+    bool vcycle64(const double* r, double* z);
+    bool vcycle32(const float* r, float* z);
+
+    // Krylov solvers on caller-ordered device vectors (FP64).
+    bool pcg(const double* b, double* x, uint64_t max_iter, double tol, SolveStats& st, std::string& err);
+    bool fgmres(const double* b, double* x, int restart, uint64_t max_iter, double tol, bool mixed,
+                SolveStats& st, std::string& err);
+    // y = A x on caller-ordered device vectors; returns seconds per launch.
+    bool spmv_caller(uint32_t precision, const void* x, void* y, uint32_t repeat, double& sec, std::string& err);
+
+    hipStream_t stream() const { return stream_; }
+    const std::vector<uint32_t>& perm0() const { return lv_[0].new_of_old; }
+
+private:
+    template <class T>
+    bool vcycle_t(uint32_t l, const T* b, T*& x);
+    template <class T>
+    void spmv_jacobi_sweeps(DevLevel& L, const T* b, T*& x, uint32_t sweeps, bool from_zero);
+    void release();
+    bool ensure_work(uint64_t n_partials);
+    void reduce(const double* partials, uint32_t n, uint32_t count, double* out);
+    bool fetch_scalars(uint32_t count);
+
+    std::vector<DevLevel> lv_;
+    DevCoarseLU lu_;
+    double omega_ = 0.7;
+    uint32_t pre_ = 6, post_ = 6;
+    bool has64_ = false, has32_ = false;
+    hipStream_t stream_ = nullptr;
+    uint32_t* d_perm0_ = nullptr;  // new_of_old of level 0 on device
+    // Krylov work (caller size n0), FP64.
+    double* partials_ = nullptr;    // 8 partial arrays
+    uint64_t partial_stride_ = 0;
+    double* scal_ = nullptr;        // device scalars
+    double* h_scal_ = nullptr;      // pinned host mirror
+    std::vector<double*> work_;     // generic FP64 n-vectors
+    std::vector<float*> work32_;
+    std::vector<double*> basis_;    // FGMRES V (m+1) and Z (m)
+    uint32_t basis_m_ = 0;
+};
+
+}  // namespace cfdxhip

@@ -1,0 +1,730 @@
+// Finite-volume geometry and pressure-system assembly on the device.
+//
+// Reference anchors (physics/steady_incompressible_solver.h unless noted):
+//   face geometry (triangle fan)     core/geometry/face_geometry.h:50-90
+//   cell centre / volume             core/geometry/cell_geometry.h:49-147
+//   build_fv_geometry (2 passes)     physics/finite_volume_transport.h:92-175
+//   gauss_gradient_with_boundary     :1634-1750
+//   make_mass_flux                   :1842-1905
+//   orthogonal coefficient           :1914-1935
+//   non-orthogonal (deferred) flux   :1941-1965
+//   Rhie-Chow internal flux          :1967-2029
+//   corrector assembly               :8048-8181, CSR + gauge :8599-8652
+//   scalar diffusion assembly        core/solvers/scalar_diffusion.h:39-163
+//
+// Face kernels compute per-face quantities once; cell kernels gather them
+// over the cell->face lists. The reference accumulates the diagonal and the
+// boundary sources in global face order; cells list their faces in ascending
+// face id, so the gather visits them in that same order and every value is
+// bit-identical to the reference (FMA contraction is off). No atomics.
+
+#include "fv.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+#include "kernels.hpp"
+
+namespace cfdxhip {
+
+#define HIP_OK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                  \
+    } while (0)
+
+namespace {
+
+template <class T>
+T* dalloc(uint64_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <class T>
+T* upload(const T* h, uint64_t n) {
+    T* d = dalloc<T>(n);
+    if (n) HIP_OK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+template <class T>
+void download(T* h, const T* d, uint64_t n) {
+    if (n && h) HIP_OK(hipMemcpy(h, d, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+inline uint32_t grid_for(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ D3 add3(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 sub3(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 mul3(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double dot3(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double mag3(D3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ D3 cross3(D3 a, D3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+struct Soa3 {
+    double *x, *y, *z;
+    __device__ D3 get(uint32_t i) const { return {x[i], y[i], z[i]}; }
+    __device__ void set(uint32_t i, D3 v) const {
+        x[i] = v.x;
+        y[i] = v.y;
+        z[i] = v.z;
+    }
+};
+
+// -------------------------------------------------------------- geometry --
+
+__global__ __launch_bounds__(kBlock) void face_geometry_kernel(uint32_t nf, const double* __restrict__ px,
+                                                              const double* __restrict__ py,
+                                                              const double* __restrict__ pz,
+                                                              const uint32_t* __restrict__ foff,
+                                                              const uint32_t* __restrict__ fvtx, Soa3 fc, Soa3 fs,
+                                                              int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const uint32_t off = foff[f], nv = foff[f + 1] - off;
+    if (nv < 3) {
+        err[0] = 1;
+        return;
+    }
+    const uint32_t v0 = fvtx[off];
+    const D3 p0 = d3(px[v0], py[v0], pz[v0]);
+    D3 S = d3(0.0, 0.0, 0.0), wc = d3(0.0, 0.0, 0.0);
+    double tri = 0.0;
+    for (uint32_t k = 1; k + 1 < nv; ++k) {
+        const uint32_t va = fvtx[off + k], vb = fvtx[off + k + 1];
+        const D3 pa = d3(px[va], py[va], pz[va]);
+        const D3 pb = d3(px[vb], py[vb], pz[vb]);
+        const D3 tS = mul3(cross3(sub3(pa, p0), sub3(pb, p0)), 0.5);
+        const double ta = mag3(tS);
+        S = add3(S, tS);
+        wc = add3(wc, mul3(add3(add3(p0, pa), pb), ta / 3.0));
+        tri += ta;
+    }
+    const double area = mag3(S);
+    if (!(tri > 0.0) || !(area > 0.0) || !isfinite(area)) err[0] = 1;
+    fc.set(f, mul3(wc, 1.0 / tri));
+    fs.set(f, S);
+}
+
+// Cell centre: area-weighted face-centre mean (orientation independent).
+__global__ __launch_bounds__(kBlock) void cell_centre_kernel(uint32_t nc, const uint32_t* __restrict__ coff,
+                                                            const uint32_t* __restrict__ cf, Soa3 fc, Soa3 fs,
+                                                            Soa3 cc) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    D3 ws = d3(0.0, 0.0, 0.0);
+    double total = 0.0;
+    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+        const uint32_t f = cf[k];
+        const double a = mag3(fs.get(f));
+        ws = add3(ws, mul3(fc.get(f), a));
+        total += a;
+    }
+    cc.set(c, total > 0.0 ? mul3(ws, 1.0 / total) : d3(0.0, 0.0, 0.0));
+}
+
+__global__ __launch_bounds__(kBlock) void orient_kernel(uint32_t nf, const uint32_t* __restrict__ owner,
+                                                       const int32_t* __restrict__ neigh, Soa3 fc, Soa3 fs,
+                                                       Soa3 cc) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const uint32_t o = owner[f];
+    const D3 d = neigh[f] >= 0 ? sub3(cc.get(uint32_t(neigh[f])), cc.get(o)) : sub3(fc.get(f), cc.get(o));
+    const D3 s = fs.get(f);
+    if (dot3(s, d) < 0.0) fs.set(f, mul3(s, -1.0));
+}
+
+__global__ __launch_bounds__(kBlock) void cell_volume_kernel(uint32_t nc, const uint32_t* __restrict__ coff,
+                                                            const uint32_t* __restrict__ cf,
+                                                            const uint32_t* __restrict__ owner, Soa3 fc, Soa3 fs,
+                                                            Soa3 cc, double* __restrict__ vol,
+                                                            int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const D3 centre = cc.get(c);
+    double sv = 0.0;
+    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+        const uint32_t f = cf[k];
+        const D3 s = owner[f] == c ? fs.get(f) : mul3(fs.get(f), -1.0);
+        sv += dot3(sub3(fc.get(f), centre), s);
+    }
+    sv /= 3.0;
+    const double v = fabs(sv);
+    if (!(v > 0.0) || !isfinite(v)) err[0] = 2;
+    vol[c] = v;
+}
+
+// ---------------------------------------------------------- gradient ----
+
+// bc arrays are per patch: type 0 none/zero-gradient, 1 fixed value,
+// 2 fixed gradient; kind 1 = EMPTY patch (face skipped).
+__global__ __launch_bounds__(kBlock) void gauss_gradient_kernel(
+    uint32_t nc, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cf,
+    const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh, const uint32_t* __restrict__ patch,
+    uint32_t npatch, const uint8_t* __restrict__ kind, const uint8_t* __restrict__ bct,
+    const double* __restrict__ bcv, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
+    const double* __restrict__ p, double* __restrict__ grad) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    D3 sum = d3(0.0, 0.0, 0.0);
+    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+        const uint32_t f = cf[k];
+        const bool own = owner[f] == c;
+        const D3 Sf = own ? fs.get(f) : mul3(fs.get(f), -1.0);
+        double vf = p[c];
+        if (neigh[f] >= 0) {
+            const uint32_t other = own ? uint32_t(neigh[f]) : owner[f];
+            const double dc = mag3(sub3(fc.get(f), cc.get(c)));
+            const double dn = mag3(sub3(cc.get(other), fc.get(f)));
+            const double w = dn / (dc + dn);
+            vf = w * p[c] + (1.0 - w) * p[other];
+        } else {
+            const uint32_t pp = patch[f];
+            if (pp < npatch && kind[pp] == 1) continue;
+            if (pp < npatch) {
+                if (bct[pp] == 1) {
+                    vf = bcv[pp];
+                } else if (bct[pp] == 2) {
+                    const D3 out = mul3(Sf, 1.0 / mag3(Sf));
+                    vf += bcv[pp] * dot3(sub3(fc.get(f), cc.get(c)), out);
+                }
+            }
+        }
+        sum = add3(sum, mul3(Sf, vf));
+    }
+    const double invV = 1.0 / vol[c];
+    grad[c] = sum.x * invV;
+    grad[nc + c] = sum.y * invV;
+    grad[2 * size_t(nc) + c] = sum.z * invV;
+}
+
+__global__ __launch_bounds__(kBlock) void mass_flux_kernel(uint32_t nf, uint32_t nc, const uint32_t* __restrict__ owner,
+                                                          const int32_t* __restrict__ neigh,
+                                                          const uint32_t* __restrict__ patch, uint32_t npatch,
+                                                          const uint8_t* __restrict__ kind,
+                                                          const uint8_t* __restrict__ vbt,
+                                                          const double* __restrict__ vbv, double rho, Soa3 fs,
+                                                          const double* __restrict__ U, double* __restrict__ phi) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const uint32_t o = owner[f];
+    D3 Uf = d3(U[o], U[nc + o], U[2 * size_t(nc) + o]);
+    if (neigh[f] >= 0) {
+        const uint32_t n = uint32_t(neigh[f]);
+        const D3 Un = d3(U[n], U[nc + n], U[2 * size_t(nc) + n]);
+        Uf = add3(mul3(Uf, 1.0 - 0.5), mul3(Un, 0.5));
+    } else {
+        const uint32_t pp = patch[f];
+        if (pp < npatch && (kind[pp] == 1 || kind[pp] == 2)) {
+            phi[f] = 0.0;
+            return;
+        }
+        if (pp < npatch && vbt[pp] == 1) Uf = d3(vbv[3 * pp], vbv[3 * pp + 1], vbv[3 * pp + 2]);
+    }
+    phi[f] = rho * dot3(Uf, fs.get(f));
+}
+
+// ---------------------------------------------------------- pressure ----
+
+__device__ __forceinline__ bool ortho_coeff(D3 Sf, D3 dv, D3 dAU, double& out) {
+#pragma clang fp contract(off)
+    const double d = mag3(dv);
+    const double area = mag3(Sf);
+    if (!(d > 0.0) || !isfinite(d) || !(area > 0.0) || !isfinite(area)) return false;
+    const D3 e = d3(dv.x / d, dv.y / d, dv.z / d);
+    const double oa = dot3(Sf, e);
+    if (!(oa > 0.0) || !isfinite(oa)) return false;
+    const double rfn = dAU.x * e.x * e.x + dAU.y * e.y * e.y + dAU.z * e.z * e.z;
+    out = rfn * oa / d;
+    return isfinite(out);
+}
+
+__device__ __forceinline__ bool nonortho_flux(D3 Sf, D3 dv, D3 dAU, D3 g, double& out) {
+#pragma clang fp contract(off)
+    const double d = mag3(dv);
+    if (!(d > 0.0) || !isfinite(d)) return false;
+    const D3 e = d3(dv.x / d, dv.y / d, dv.z / d);
+    const double oa = dot3(Sf, e);
+    const double rfn = dAU.x * e.x * e.x + dAU.y * e.y * e.y + dAU.z * e.z * e.z;
+    const double full = dAU.x * g.x * Sf.x + dAU.y * g.y * Sf.y + dAU.z * g.z * Sf.z;
+    const double impl = rfn * dot3(g, e) * oa;
+    out = full - impl;
+    return isfinite(out);
+}
+
+// Per face: internal -> coeff (rho * orthogonal coefficient) and the
+// Rhie-Chow pressure flux; fixed-pressure boundary -> boundary coefficient
+// and source. Weights are the default w = 1/2.
+__global__ __launch_bounds__(kBlock) void pressure_face_kernel(
+    uint32_t nf, uint32_t nc, const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh,
+    const uint32_t* __restrict__ patch, uint32_t npatch, const uint8_t* __restrict__ pbt,
+    const double* __restrict__ pbv, double rho, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
+    const double* __restrict__ rAU, const double* __restrict__ p, const double* __restrict__ grad,
+    double* __restrict__ coeff, double* __restrict__ rcflux, double* __restrict__ bsrc, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const uint32_t o = owner[f];
+    const size_t n2 = 2 * size_t(nc);
+    coeff[f] = 0.0;
+    rcflux[f] = 0.0;
+    bsrc[f] = 0.0;
+    if (neigh[f] >= 0) {
+        const uint32_t n = uint32_t(neigh[f]);
+        const double wn = 0.5, wo = 1.0 - wn;
+        const D3 dAU = d3(wo * vol[o] * rAU[o] + wn * vol[n] * rAU[n],
+                          wo * vol[o] * rAU[nc + o] + wn * vol[n] * rAU[nc + n],
+                          wo * vol[o] * rAU[n2 + o] + wn * vol[n] * rAU[n2 + n]);
+        const D3 gp = d3(wo * grad[o] + wn * grad[n], wo * grad[nc + o] + wn * grad[nc + n],
+                         wo * grad[n2 + o] + wn * grad[n2 + n]);
+        const D3 Sf = fs.get(f);
+        const D3 dv = sub3(cc.get(n), cc.get(o));
+        double ic = 0.0, nof = 0.0;
+        if (!ortho_coeff(Sf, dv, dAU, ic) || !nonortho_flux(Sf, dv, dAU, gp, nof)) {
+            err[0] = 3;
+            return;
+        }
+        coeff[f] = rho * ic;
+        rcflux[f] = rho * (ic * (p[n] - p[o]) + nof);
+        return;
+    }
+    const uint32_t pp = patch[f];
+    if (pp >= npatch || pbt[pp] != 1) return;
+    const D3 dv = sub3(fc.get(f), cc.get(o));
+    const D3 Sf = fs.get(f);
+    const D3 dAU = d3(vol[o] * rAU[o], vol[o] * rAU[nc + o], vol[o] * rAU[n2 + o]);
+    double bc = 0.0, def = 0.0;
+    const D3 gp = d3(grad[o], grad[nc + o], grad[n2 + o]);
+    if (!ortho_coeff(Sf, dv, dAU, bc) || !nonortho_flux(Sf, dv, dAU, gp, def)) {
+        err[0] = 3;
+        return;
+    }
+    coeff[f] = rho * bc;
+    bsrc[f] = rho * (bc * (pbv[pp] - p[o]) + def);
+}
+
+// Per cell: continuity in cell-face order (±phiHbyA, ∓RC flux), diagonal =
+// internal coefficients then fixed-pressure boundary coefficients, both in
+// ascending face order; rhs = -continuity + boundary sources.
+__global__ __launch_bounds__(kBlock) void pressure_cell_kernel(
+    uint32_t nc, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cf,
+    const uint32_t* __restrict__ cfs, const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh,
+    const double* __restrict__ phi, const double* __restrict__ coeff, const double* __restrict__ rcflux,
+    const double* __restrict__ bsrc, const uint8_t* __restrict__ fixed_face, double* __restrict__ diag,
+    double* __restrict__ rhs) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    double cont = 0.0;
+    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+        const uint32_t f = cf[k];
+        const bool own = owner[f] == c;
+        cont += own ? phi[f] : -phi[f];
+        if (neigh[f] >= 0) cont += own ? -rcflux[f] : rcflux[f];
+    }
+    double d = 0.0;
+    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+        const uint32_t f = cfs[k];
+        if (neigh[f] >= 0) d += coeff[f];
+    }
+    double dd = 0.0 + d;
+    double b = -cont;
+    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+        const uint32_t f = cfs[k];
+        if (neigh[f] < 0 && fixed_face[f]) {
+            dd += coeff[f];
+            b += bsrc[f];
+        }
+    }
+    diag[c] = dd;
+    rhs[c] = b;
+}
+
+// CSR values of the full system: diagonal from diag[], off-diagonals
+// 0 - coeff(f1) - coeff(f2) ... over the faces joining the pair.
+__global__ __launch_bounds__(kBlock) void csr_values_kernel(uint64_t nnz, const uint32_t* __restrict__ nz_row_diag,
+                                                           const uint32_t* __restrict__ nzf_off,
+                                                           const uint32_t* __restrict__ nzf,
+                                                           const double* __restrict__ diag,
+                                                           const double* __restrict__ coeff,
+                                                           double* __restrict__ vals) {
+#pragma clang fp contract(off)
+    const uint64_t k = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (k >= nnz) return;
+    const uint32_t dr = nz_row_diag[k];
+    if (dr != 0xffffffffu) {
+        vals[k] = diag[dr];
+        return;
+    }
+    double v = 0.0;
+    for (uint32_t q = nzf_off[k]; q < nzf_off[k + 1]; ++q) v -= coeff[nzf[q]];
+    vals[k] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void gather_u32idx_kernel(uint64_t n, const uint32_t* __restrict__ src_idx,
+                                                              const double* __restrict__ src,
+                                                              double* __restrict__ dst) {
+    const uint64_t k = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (k < n) dst[k] = src[src_idx[k]];
+}
+
+// ---------------------------------------------------------- diffusion ---
+
+__global__ __launch_bounds__(kBlock) void diffusion_face_kernel(uint32_t nf, const uint32_t* __restrict__ owner,
+                                                               const int32_t* __restrict__ neigh, double gamma,
+                                                               Soa3 fc, Soa3 fs, Soa3 cc,
+                                                               const double* __restrict__ fval,
+                                                               double* __restrict__ g, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const uint32_t o = owner[f];
+    const double area = mag3(fs.get(f));
+    if (!(area > 0.0)) err[0] = 4;
+    g[f] = 0.0;
+    double d = 0.0;
+    if (neigh[f] >= 0) {
+        d = mag3(sub3(cc.get(uint32_t(neigh[f])), cc.get(o)));
+    } else if (isfinite(fval[f])) {
+        d = mag3(sub3(fc.get(f), cc.get(o)));
+    } else {
+        return;
+    }
+    if (!(d > 2.220446049250313e-16)) {
+        err[0] = 4;
+        return;
+    }
+    g[f] = gamma * area / d;
+}
+
+__global__ __launch_bounds__(kBlock) void diffusion_cell_kernel(uint32_t nc, const uint32_t* __restrict__ coff,
+                                                               const uint32_t* __restrict__ cfs,
+                                                               const int32_t* __restrict__ neigh, Soa3 fs,
+                                                               const double* __restrict__ vol,
+                                                               const double* __restrict__ source,
+                                                               const double* __restrict__ fval,
+                                                               const uint8_t* __restrict__ fneu,
+                                                               const double* __restrict__ g,
+                                                               double* __restrict__ diag, double* __restrict__ rhs) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    double d = 0.0;
+    double b = source[c] * vol[c];
+    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+        const uint32_t f = cfs[k];
+        if (neigh[f] >= 0) {
+            d += g[f];
+        } else if (isfinite(fval[f])) {
+            if (!fneu[f]) {
+                d += g[f];
+                b += g[f] * fval[f];
+            } else {
+                b += fval[f] * mag3(fs.get(f));
+            }
+        }
+    }
+    diag[c] = 0.0 + d;
+    rhs[c] = b;
+}
+
+__global__ __launch_bounds__(kBlock) void soa_to_aos_kernel(uint32_t n, Soa3 s, double* __restrict__ aos) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    aos[3 * size_t(i)] = s.x[i];
+    aos[3 * size_t(i) + 1] = s.y[i];
+    aos[3 * size_t(i) + 2] = s.z[i];
+}
+
+}  // namespace
+
+// ==================================================================== FvMesh
+
+FvMesh::~FvMesh() { release(); }
+
+void FvMesh::release() {
+    for (double** p : {&px, &py, &pz, &fcx, &fcy, &fcz, &fsx, &fsy, &fsz, &ccx, &ccy, &ccz, &vol, &work_coeff,
+                       &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi})
+        dfree(*p);
+    for (uint32_t** p : {&foff, &fvtx, &owner, &coff, &cfaces, &cfaces_sorted, &patch})
+        dfree(*p);
+    dfree(neigh);
+    dfree(patch_kind);
+    dfree(err);
+    for (auto* pl : {&pressure_full, &pressure_red})
+        for (uint32_t** p : {&pl->rowp, &pl->cols, &pl->nz_row_diag, &pl->nzf_off, &pl->nzf, &pl->red_src,
+                             &pl->red_rhs_src})
+            dfree(*p);
+    dfree(fixed_face);
+}
+
+void FvMesh::create(int device, uint64_t np_, const double* hx, const double* hy, const double* hz, uint64_t nf_,
+                    const uint32_t* hfoff, const uint32_t* hfvtx, const uint32_t* hown, const int32_t* hnei,
+                    uint64_t nc_, const uint32_t* hcoff, const uint32_t* hcf, uint64_t npatch_,
+                    const uint32_t* hpatch, const uint8_t* hkind) {
+    if (nf_ >= UINT32_MAX || nc_ >= UINT32_MAX || np_ >= UINT32_MAX) throw std::invalid_argument("mesh too large");
+    dev = device;
+    HIP_OK(hipSetDevice(dev));
+    np = uint32_t(np_);
+    nf = uint32_t(nf_);
+    nc = uint32_t(nc_);
+    npatch = uint32_t(npatch_);
+    // Host copies needed to build sparsity plans.
+    h_owner.assign(hown, hown + nf);
+    h_neigh.assign(hnei, hnei + nf);
+    h_coff.assign(hcoff, hcoff + nc + 1);
+    h_cf.assign(hcf, hcf + h_coff[nc]);
+    h_patch.assign(hpatch, hpatch + nf);
+    h_kind.assign(hkind, hkind + npatch);
+    for (uint32_t f = 0; f < nf; ++f) {
+        if (h_owner[f] >= nc || h_neigh[f] >= int64_t(nc)) throw std::invalid_argument("face addressing out of range");
+    }
+    for (uint32_t k = 0; k < h_coff[nc]; ++k)
+        if (h_cf[k] >= nf) throw std::invalid_argument("cell face index out of range");
+    // Face-id-ascending copy of each cell's face list (global face order).
+    std::vector<uint32_t> sorted(h_cf);
+    bool already = true;
+    for (uint32_t c = 0; c < nc; ++c) {
+        std::sort(sorted.begin() + h_coff[c], sorted.begin() + h_coff[c + 1]);
+        for (uint32_t k = h_coff[c]; k < h_coff[c + 1]; ++k) already = already && sorted[k] == h_cf[k];
+    }
+    px = upload(hx, np);
+    py = upload(hy, np);
+    pz = upload(hz, np);
+    foff = upload(hfoff, uint64_t(nf) + 1);
+    fvtx = upload(hfvtx, uint64_t(hfoff[nf]));
+    for (uint32_t k = 0; k < hfoff[nf]; ++k)
+        if (hfvtx[k] >= np) throw std::invalid_argument("face vertex out of range");
+    owner = upload(hown, nf);
+    neigh = upload(hnei, nf);
+    coff = upload(hcoff, uint64_t(nc) + 1);
+    cfaces = upload(hcf, uint64_t(h_coff[nc]));
+    cfaces_sorted = already ? nullptr : upload(sorted.data(), sorted.size());
+    patch = upload(hpatch, nf);
+    patch_kind = upload(hkind, npatch);
+    for (double** p : {&fcx, &fcy, &fcz, &fsx, &fsy, &fsz}) *p = dalloc<double>(nf);
+    for (double** p : {&ccx, &ccy, &ccz, &vol}) *p = dalloc<double>(nc);
+    err = dalloc<int>(4);
+    HIP_OK(hipStreamCreate(&stream));
+    compute_geometry();
+}
+
+void FvMesh::check_err(const char* what) {
+    int h[4] = {0, 0, 0, 0};
+    HIP_OK(hipMemcpy(h, err, sizeof h, hipMemcpyDeviceToHost));
+    if (h[0] != 0) {
+        HIP_OK(hipMemset(err, 0, 4 * sizeof(int)));
+        throw std::domain_error(std::string(what) + " (code " + std::to_string(h[0]) + ")");
+    }
+}
+
+void FvMesh::compute_geometry() {
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    face_geometry_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, px, py, pz, foff, fvtx, fc, fs, err);
+    cell_centre_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, fc, fs, cc);
+    orient_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, fc, fs, cc);
+    cell_volume_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, fc, fs, cc, vol, err);
+    HIP_OK(hipStreamSynchronize(stream));
+    check_err("invalid face or cell geometry");
+}
+
+void FvMesh::geometry_out(double* fc_aos, double* fs_aos, double* cc_aos, double* vol_out) {
+    double* tmp = dalloc<double>(3 * uint64_t(std::max(nf, nc)));
+    auto out3 = [&](Soa3 s, uint32_t n, double* h) {
+        if (!h) return;
+        soa_to_aos_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, s, tmp);
+        HIP_OK(hipStreamSynchronize(stream));
+        download(h, tmp, 3 * uint64_t(n));
+    };
+    out3(Soa3{fcx, fcy, fcz}, nf, fc_aos);
+    out3(Soa3{fsx, fsy, fsz}, nf, fs_aos);
+    out3(Soa3{ccx, ccy, ccz}, nc, cc_aos);
+    download(vol_out, vol, nc);
+    dfree(tmp);
+}
+
+void FvMesh::ensure_work() {
+    if (work_coeff) return;
+    work_coeff = dalloc<double>(nf);
+    work_rc = dalloc<double>(nf);
+    work_bsrc = dalloc<double>(nf);
+    work_phi = dalloc<double>(nf);
+    work_diag = dalloc<double>(nc);
+    work_rhs = dalloc<double>(nc);
+    work_grad = dalloc<double>(3 * uint64_t(nc));
+    fixed_face = dalloc<uint8_t>(nf);
+}
+
+void FvMesh::gauss_gradient_device(const uint8_t* d_bct, const double* d_bcv, const double* d_p, double* d_grad) {
+    const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    gauss_gradient_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
+                                                               patch_kind, d_bct, d_bcv, fc, fs, cc, vol, d_p,
+                                                               d_grad);
+}
+
+void FvMesh::mass_flux_device(const uint8_t* d_vbt, const double* d_vbv, double rho, const double* d_U,
+                              double* d_phi) {
+    const Soa3 fs{fsx, fsy, fsz};
+    mass_flux_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind, d_vbt,
+                                                          d_vbv, rho, fs, d_U, d_phi);
+}
+
+// Sparsity of the pressure system: diagonal + face neighbours, sorted per
+// row (sparse_matrix.h:76-131 canonical order); with reference-cell
+// elimination the row/column `ref` is dropped and indices above it shift.
+const PressurePlan& FvMesh::pressure_plan(bool eliminate, uint32_t ref) {
+    PressurePlan& full = pressure_full;
+    if (!full.rowp) {
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> rows(nc);  // (col, face)
+        for (uint32_t f = 0; f < nf; ++f) {
+            if (h_neigh[f] < 0) continue;
+            const uint32_t o = h_owner[f], n = uint32_t(h_neigh[f]);
+            rows[o].push_back({n, f});
+            rows[n].push_back({o, f});
+        }
+        std::vector<uint32_t> rowp(uint64_t(nc) + 1, 0), cols, rdiag, nzf_off{0}, nzf;
+        for (uint32_t c = 0; c < nc; ++c) {
+            auto& r = rows[c];
+            r.push_back({c, 0xffffffffu});
+            std::sort(r.begin(), r.end());
+            for (size_t q = 0; q < r.size();) {
+                const uint32_t col = r[q].first;
+                cols.push_back(col);
+                rdiag.push_back(col == c ? c : 0xffffffffu);
+                while (q < r.size() && r[q].first == col) {
+                    if (r[q].second != 0xffffffffu) nzf.push_back(r[q].second);
+                    ++q;
+                }
+                nzf_off.push_back(uint32_t(nzf.size()));
+            }
+            rowp[c + 1] = uint32_t(cols.size());
+            std::vector<std::pair<uint32_t, uint32_t>>().swap(r);
+        }
+        full.n = nc;
+        full.nnz = cols.size();
+        full.h_rowp = rowp;
+        full.h_cols = cols;
+        full.rowp = upload(rowp.data(), rowp.size());
+        full.cols = upload(cols.data(), cols.size());
+        full.nz_row_diag = upload(rdiag.data(), rdiag.size());
+        full.nzf_off = upload(nzf_off.data(), nzf_off.size());
+        full.nzf = upload(nzf.data(), nzf.size());
+    }
+    if (!eliminate) return full;
+    PressurePlan& red = pressure_red;
+    if (!red.rowp || red.ref != ref) {
+        for (uint32_t** p : {&red.rowp, &red.cols, &red.red_src, &red.red_rhs_src}) dfree(*p);
+        std::vector<uint32_t> rowp{0}, cols, src, rsrc;
+        for (uint32_t c = 0; c < nc; ++c) {
+            if (c == ref) continue;
+            rsrc.push_back(c);
+            for (uint32_t k = full.h_rowp[c]; k < full.h_rowp[c + 1]; ++k) {
+                const uint32_t j = full.h_cols[k];
+                if (j == ref) continue;
+                cols.push_back(j < ref ? j : j - 1);
+                src.push_back(k);
+            }
+            rowp.push_back(uint32_t(cols.size()));
+        }
+        red.n = nc - 1;
+        red.nnz = cols.size();
+        red.ref = ref;
+        red.h_rowp = rowp;
+        red.h_cols = cols;
+        red.rowp = upload(rowp.data(), rowp.size());
+        red.cols = upload(cols.data(), cols.size());
+        red.red_src = upload(src.data(), src.size());
+        red.red_rhs_src = upload(rsrc.data(), rsrc.size());
+    }
+    return red;
+}
+
+bool FvMesh::has_fixed_pressure(const uint8_t* h_pbt) const {
+    for (uint32_t f = 0; f < nf; ++f)
+        if (h_neigh[f] < 0 && h_patch[f] < npatch && h_pbt[h_patch[f]] == 1) return true;
+    return false;
+}
+
+const PressurePlan& FvMesh::pressure_assemble_device(const uint8_t* h_pbt, const double* h_pbv, double rho,
+                                                     const double* d_rAU, const double* d_phi, const double* d_p,
+                                                     uint64_t ref, double* d_vals, double* d_rhs) {
+    ensure_work();
+    const bool fixed = has_fixed_pressure(h_pbt);
+    const bool eliminate = !fixed && nc > 1 && ref < nc;
+    const PressurePlan& plan = pressure_plan(eliminate, uint32_t(eliminate ? ref : 0));
+    uint8_t* d_pbt = upload(h_pbt, npatch);
+    double* d_pbv = upload(h_pbv, npatch);
+    std::vector<uint8_t> ff(nf, 0);
+    for (uint32_t f = 0; f < nf; ++f) ff[f] = h_neigh[f] < 0 && h_patch[f] < npatch && h_pbt[h_patch[f]] == 1;
+    HIP_OK(hipMemcpy(fixed_face, ff.data(), nf, hipMemcpyHostToDevice));
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    gauss_gradient_device(d_pbt, d_pbv, d_p, work_grad);
+    pressure_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, d_pbt, d_pbv, rho,
+                                                              fc, fs, cc, vol, d_rAU, d_p, work_grad, work_coeff,
+                                                              work_rc, work_bsrc, err);
+    pressure_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, cfaces_sorted ? cfaces_sorted : cfaces,
+                                                              owner, neigh, d_phi, work_coeff, work_rc, work_bsrc,
+                                                              fixed_face, work_diag, work_rhs);
+    if (!eliminate) {
+        csr_values_kernel<<<grid_for(plan.nnz), kBlock, 0, stream>>>(plan.nnz, pressure_full.nz_row_diag,
+                                                                     pressure_full.nzf_off, pressure_full.nzf,
+                                                                     work_diag, work_coeff, d_vals);
+        HIP_OK(hipMemcpyAsync(d_rhs, work_rhs, uint64_t(nc) * 8, hipMemcpyDeviceToDevice, stream));
+    } else {
+        if (!work_vals) work_vals = dalloc<double>(pressure_full.nnz);
+        csr_values_kernel<<<grid_for(pressure_full.nnz), kBlock, 0, stream>>>(
+            pressure_full.nnz, pressure_full.nz_row_diag, pressure_full.nzf_off, pressure_full.nzf, work_diag,
+            work_coeff, work_vals);
+        gather_u32idx_kernel<<<grid_for(plan.nnz), kBlock, 0, stream>>>(plan.nnz, plan.red_src, work_vals, d_vals);
+        gather_u32idx_kernel<<<grid_for(plan.n), kBlock, 0, stream>>>(plan.n, plan.red_rhs_src, work_rhs, d_rhs);
+    }
+    HIP_OK(hipStreamSynchronize(stream));
+    dfree(d_pbt);
+    dfree(d_pbv);
+    check_err("pressure_face_orthogonal_coefficient: invalid face geometry or orientation");
+    return plan;
+}
+
+const PressurePlan& FvMesh::diffusion_assemble_device(double gamma, const double* d_source, const double* d_fval,
+                                                      const uint8_t* d_fneu, double* d_vals, double* d_rhs) {
+    ensure_work();
+    const PressurePlan& plan = pressure_plan(false, 0);
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    diffusion_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, gamma, fc, fs, cc, d_fval,
+                                                               work_coeff, err);
+    diffusion_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces_sorted ? cfaces_sorted : cfaces,
+                                                               neigh, fs, vol, d_source, d_fval, d_fneu, work_coeff,
+                                                               work_diag, d_rhs);
+    csr_values_kernel<<<grid_for(plan.nnz), kBlock, 0, stream>>>(plan.nnz, plan.nz_row_diag, plan.nzf_off, plan.nzf,
+                                                                 work_diag, work_coeff, d_vals);
+    HIP_OK(hipStreamSynchronize(stream));
+    check_err("assemble_cell_diffusion_matrix: invalid geometry");
+    return plan;
+}
+
+}  // namespace cfdxhip

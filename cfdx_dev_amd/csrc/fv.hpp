@@ -1,0 +1,80 @@
+// Device-resident finite-volume mesh, geometry and pressure assembly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace cfdxhip {
+
+// CSR sparsity of an assembled system and the maps that fill its values.
+struct PressurePlan {
+    uint32_t n = 0;
+    uint64_t nnz = 0;
+    uint32_t ref = 0;
+    std::vector<uint32_t> h_rowp, h_cols;
+    uint32_t* rowp = nullptr;
+    uint32_t* cols = nullptr;
+    // full system only: per nonzero, the diagonal's row (or ~0u) and the
+    // faces joining the pair, ascending
+    uint32_t* nz_row_diag = nullptr;
+    uint32_t* nzf_off = nullptr;
+    uint32_t* nzf = nullptr;
+    // reduced system only: source nonzero / source row in the full system
+    uint32_t* red_src = nullptr;
+    uint32_t* red_rhs_src = nullptr;
+};
+
+struct FvMesh {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    uint32_t np = 0, nf = 0, nc = 0, npatch = 0;
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    uint32_t *foff = nullptr, *fvtx = nullptr, *owner = nullptr, *coff = nullptr, *cfaces = nullptr;
+    uint32_t* cfaces_sorted = nullptr;  // null when cfaces is already ascending per cell
+    int32_t* neigh = nullptr;
+    uint32_t* patch = nullptr;
+    uint8_t* patch_kind = nullptr;
+    // geometry (SoA)
+    double *fcx = nullptr, *fcy = nullptr, *fcz = nullptr;
+    double *fsx = nullptr, *fsy = nullptr, *fsz = nullptr;
+    double *ccx = nullptr, *ccy = nullptr, *ccz = nullptr;
+    double* vol = nullptr;
+    int* err = nullptr;
+    // work
+    double *work_coeff = nullptr, *work_rc = nullptr, *work_bsrc = nullptr, *work_diag = nullptr,
+           *work_rhs = nullptr, *work_vals = nullptr, *work_grad = nullptr, *work_phi = nullptr;
+    uint8_t* fixed_face = nullptr;
+    PressurePlan pressure_full, pressure_red;
+    // host copies of the addressing
+    std::vector<uint32_t> h_owner, h_coff, h_cf, h_patch;
+    std::vector<int32_t> h_neigh;
+    std::vector<uint8_t> h_kind;
+
+    FvMesh() = default;
+    ~FvMesh();
+    FvMesh(const FvMesh&) = delete;
+    FvMesh& operator=(const FvMesh&) = delete;
+
+    void create(int device, uint64_t np, const double* px, const double* py, const double* pz, uint64_t nf,
+                const uint32_t* foff, const uint32_t* fvtx, const uint32_t* owner, const int32_t* neigh,
+                uint64_t nc, const uint32_t* coff, const uint32_t* cf, uint64_t npatch, const uint32_t* patch,
+                const uint8_t* kind);
+    void compute_geometry();
+    void geometry_out(double* fc_aos, double* fs_aos, double* cc_aos, double* vol);
+    void gauss_gradient_device(const uint8_t* d_bct, const double* d_bcv, const double* d_p, double* d_grad);
+    void mass_flux_device(const uint8_t* d_vbt, const double* d_vbv, double rho, const double* d_U, double* d_phi);
+    const PressurePlan& pressure_plan(bool eliminate, uint32_t ref);
+    bool has_fixed_pressure(const uint8_t* h_pbt) const;
+    const PressurePlan& pressure_assemble_device(const uint8_t* h_pbt, const double* h_pbv, double rho,
+                                                 const double* d_rAU, const double* d_phi, const double* d_p,
+                                                 uint64_t ref, double* d_vals, double* d_rhs);
+    const PressurePlan& diffusion_assemble_device(double gamma, const double* d_source, const double* d_fval,
+                                                  const uint8_t* d_fneu, double* d_vals, double* d_rhs);
+    void ensure_work();
+    void check_err(const char* what);
+    void release();
+};
+
+}  // namespace cfdxhip

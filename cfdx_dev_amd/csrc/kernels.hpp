@@ -1,0 +1,328 @@
+// Device kernels for the sparse solve (included by .hip translation units).
+//
+// CSR-stream SpMV: one 256-thread workgroup per row block. Phase 1 streams
+// the block's values and column indices with unit stride (coalesced) and
+// stages val * x[col] in LDS; phase 2 gives each row to one thread, which sums
+// its products sequentially in storage order. The per-row order matches the
+// reference's scalar CSR loop (sparse_matrix.h:182-197), so FP64 results are
+// bit-identical to it. Epilogue functors fuse the Jacobi update, residual,
+// prolongation and dot-product partials into the same pass.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs; xcd_block() gives
+// every XCD one contiguous range of row blocks so each XCD's L2 sees a
+// contiguous window of x.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cfdxhip {
+
+constexpr int kBlock = 256;
+constexpr int kStreamNnz = 2048;  // staged products per row block
+constexpr int kMaxRowsPerBlock = 1024;
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nb) {
+    const uint32_t xcd = bid % kXcds, local = bid / kXcds;
+    const uint32_t per = nb / kXcds, extra = nb % kXcds;
+    return xcd * per + (xcd < extra ? xcd : extra) + local;
+}
+
+// ---------------------------------------------------------------- reduce --
+
+// Deterministic wavefront sum (fixed butterfly order).
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum for 256 threads: wave sums, then a fixed-order sum of 4 waves.
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T* smem4) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) smem4[wid] = v;
+    __syncthreads();
+    T r = T(0);
+    if (threadIdx.x == 0) r = ((smem4[0] + smem4[1]) + smem4[2]) + smem4[3];
+    __syncthreads();
+    return r;
+}
+
+// Sums `count` partial arrays (each of length n, stride `stride`) into
+// out[slot0 + j]. One workgroup, fixed order: deterministic.
+static __global__ __launch_bounds__(1024) void reduce_partials_kernel(const double* __restrict__ partials,
+                                                               uint32_t n, uint32_t stride, uint32_t count,
+                                                               double* __restrict__ out) {
+    __shared__ double sm[16];
+    for (uint32_t j = 0; j < count; ++j) {
+        const double* p = partials + size_t(j) * stride;
+        double v = 0.0;
+        for (uint32_t i = threadIdx.x; i < n; i += 1024) v += p[i];
+        v = wave_sum(v);
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0) sm[wid] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < 16; ++w) s += sm[w];
+            out[j] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ SpMV --
+
+template <class T, class Init, class Epi>
+__global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __restrict__ blocks, uint32_t nb,
+                                                             const uint32_t* __restrict__ rowp,
+                                                             const uint32_t* __restrict__ cols,
+                                                             const T* __restrict__ vals, const T* __restrict__ x,
+                                                             Init init, Epi epi) {
+#pragma clang fp contract(off)
+    __shared__ T prod[kStreamNnz];
+    __shared__ double red[4];
+    const uint32_t b = xcd_block(blockIdx.x, nb);
+    const uint32_t r0 = blocks[b], r1 = blocks[b + 1];
+    const uint32_t k0 = rowp[r0], k1 = rowp[r1];
+    double part = 0.0;
+    if (k1 - k0 <= uint32_t(kStreamNnz)) {
+        for (uint32_t k = k0 + threadIdx.x; k < k1; k += kBlock) prod[k - k0] = vals[k] * x[cols[k]];
+        __syncthreads();
+        for (uint32_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
+            T s = init(row);
+            const uint32_t e = rowp[row + 1];
+            for (uint32_t k = rowp[row]; k < e; ++k) s += prod[k - k0];
+            part += epi(row, s);
+        }
+    } else {
+        // One long row: stream it in chunks, a single thread keeps the
+        // sequential sum.
+        T s = init(r0);
+        for (uint32_t base = k0; base < k1; base += kStreamNnz) {
+            const uint32_t len = (k1 - base) < uint32_t(kStreamNnz) ? (k1 - base) : uint32_t(kStreamNnz);
+            for (uint32_t k = threadIdx.x; k < len; k += kBlock) prod[k] = vals[base + k] * x[cols[base + k]];
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (uint32_t k = 0; k < len; ++k) s += prod[k];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part = epi(r0, s);
+    }
+    if (Epi::kPartial) {
+        const double bs = block_sum(part, red);
+        if (threadIdx.x == 0) epi.store_partial(blockIdx.x, bs);
+    }
+}
+
+template <class T>
+struct InitZero {
+    __device__ T operator()(uint32_t) const { return T(0); }
+};
+template <class T>
+struct InitFrom {
+    const T* v;
+    __device__ T operator()(uint32_t row) const { return v[row]; }
+};
+
+// y = A x
+template <class T>
+struct EpiStore {
+    static constexpr bool kPartial = false;
+    T* y;
+    __device__ double operator()(uint32_t row, T s) const {
+        y[row] = s;
+        return 0.0;
+    }
+    __device__ void store_partial(uint32_t, double) const {}
+};
+
+// y = A x, partial += w[row] * y[row]   (p^T A p)
+template <class T>
+struct EpiStoreDot {
+    static constexpr bool kPartial = true;
+    T* y;
+    const T* w;
+    double* partial;
+    __device__ double operator()(uint32_t row, T s) const {
+        y[row] = s;
+        return double(w[row]) * double(s);
+    }
+    __device__ void store_partial(uint32_t b, double v) const { partial[b] = v; }
+};
+
+// r = b - A x, partial += r^2 (optional)
+template <class T, bool kNorm>
+struct EpiResidual {
+    static constexpr bool kPartial = kNorm;
+    const T* b;
+    T* r;
+    double* partial;
+    __device__ double operator()(uint32_t row, T s) const {
+#pragma clang fp contract(off)
+        const T v = b[row] - s;
+        r[row] = v;
+        return kNorm ? double(v) * double(v) : 0.0;
+    }
+    __device__ void store_partial(uint32_t bb, double v) const { partial[bb] = v; }
+};
+
+// Damped Jacobi sweep (amg_preconditioner.h:2394-2405):
+// xn = x + omega * (inv_diag * (b - A x))
+template <class T>
+struct EpiJacobi {
+    static constexpr bool kPartial = false;
+    const T* b;
+    const T* x;
+    const T* inv_diag;
+    T* xn;
+    T omega;
+    __device__ double operator()(uint32_t row, T s) const {
+#pragma clang fp contract(off)
+        const T delta = inv_diag[row] * (b[row] - s);
+        xn[row] = x[row] + omega * delta;
+        return 0.0;
+    }
+    __device__ void store_partial(uint32_t, double) const {}
+};
+
+// Prolongation x_i += sum_c P_ic xc_c accumulated onto x_i in entry order
+// (amg_preconditioner.h:2432-2436): init = x, store back.
+template <class T>
+struct EpiAssign {
+    static constexpr bool kPartial = false;
+    T* y;
+    __device__ double operator()(uint32_t row, T s) const {
+        y[row] = s;
+        return 0.0;
+    }
+    __device__ void store_partial(uint32_t, double) const {}
+};
+
+// -------------------------------------------------------------- vectors --
+
+// First sweep from x = 0: A*0 = 0, so x = omega * (inv_diag * b).
+template <class T>
+__global__ __launch_bounds__(kBlock) void jacobi_first_kernel(uint32_t n, const T* __restrict__ b,
+                                                             const T* __restrict__ inv_diag, T omega,
+                                                             T* __restrict__ x) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = omega * (inv_diag[i] * b[i]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void fill_kernel(uint32_t n, T v, T* __restrict__ x) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+template <class S, class D>
+__global__ __launch_bounds__(kBlock) void convert_kernel(uint32_t n, const S* __restrict__ s, D* __restrict__ d) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) d[i] = D(s[i]);
+}
+
+// dst[new_of_old[i]] = src[i]  (caller ordering -> solve ordering)
+template <class S, class D>
+__global__ __launch_bounds__(kBlock) void scatter_perm_kernel(uint32_t n, const uint32_t* __restrict__ new_of_old,
+                                                             const S* __restrict__ src, D* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[new_of_old[i]] = D(src[i]);
+}
+
+// dst[i] = src[new_of_old[i]]  (solve ordering -> caller ordering)
+template <class S, class D>
+__global__ __launch_bounds__(kBlock) void gather_perm_kernel(uint32_t n, const uint32_t* __restrict__ new_of_old,
+                                                            const S* __restrict__ src, D* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = D(src[new_of_old[i]]);
+}
+
+// Partial dot products a.b over grid-stride chunks.
+template <class T>
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(uint32_t n, const T* __restrict__ a,
+                                                            const T* __restrict__ b, double* __restrict__ partial) {
+    __shared__ double red[4];
+    double v = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        v += double(a[i]) * double(b[i]);
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// PCG update (cg_solver.h:285-294): alpha = rsold / pAp; the update is
+// skipped when the curvature or alpha is invalid (the solver then stops with
+// x untouched, as the reference does). partial += r^2.
+static __global__ __launch_bounds__(kBlock) void cg_update_kernel(uint32_t n, double rsold, const double* __restrict__ pAp_ptr,
+                                                          const double* __restrict__ p, const double* __restrict__ Ap,
+                                                          double* __restrict__ x, double* __restrict__ r,
+                                                          double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double red[4];
+    const double pAp = *pAp_ptr;
+    const double alpha = rsold / pAp;
+    const bool ok = pAp > 0.0 && isfinite(pAp) && isfinite(alpha);
+    double v = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        double ri = r[i];
+        if (ok) {
+            x[i] += alpha * p[i];
+            ri -= alpha * Ap[i];
+            r[i] = ri;
+        }
+        v += ri * ri;
+    }
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// p = z + beta p
+static __global__ __launch_bounds__(kBlock) void cg_direction_kernel(uint32_t n, double beta, const double* __restrict__ z,
+                                                             double* __restrict__ p) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void copy_kernel(uint32_t n, const T* __restrict__ s, T* __restrict__ d) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// -------------------------------------------------------- coarse solve --
+
+// Replays the dense partial-pivot elimination of the coarsest level
+// (amg_preconditioner.h:2603-2663) with factors computed once at setup.
+// One thread; n <= 16.
+template <class T>
+__global__ void coarse_lu_kernel(uint32_t n, const uint32_t* __restrict__ piv, const double* __restrict__ factor,
+                                 const double* __restrict__ upper, const T* __restrict__ r, T* __restrict__ x) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double b[16], s[16];
+    for (uint32_t i = 0; i < n; ++i) b[i] = double(r[i]);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t p = piv[k];
+        if (p != k) {
+            const double t = b[k];
+            b[k] = b[p];
+            b[p] = t;
+        }
+        for (uint32_t i = k + 1; i < n; ++i) b[i] -= factor[k * n + i] * b[k];
+    }
+    for (uint32_t ii = n; ii-- > 0;) {
+        double sum = b[ii];
+        for (uint32_t j = ii + 1; j < n; ++j) sum -= upper[ii * n + j] * s[j];
+        s[ii] = sum / upper[ii * n + ii];
+    }
+    for (uint32_t i = 0; i < n; ++i) x[i] = T(s[i]);
+}
+
+}  // namespace cfdxhip

@@ -1,0 +1,240 @@
+/*
+ * cfdxhip — MI355X-native pressure-correction solve for CFDX (C ABI).
+ *
+ * Drop-in boundary for the reference's pressure path:
+ *   - the linear-solver interface (Preconditioner / NativeAMGPreconditioner /
+ *     solve_cg_prepared / solve_fgmres), core/linalg/preconditioner.h:18-28,
+ *     core/linalg/hypre_amg.h:26-101, core/linalg/cg_solver.h:653-667,
+ *     core/linalg/gmres_solver.h:678-692;
+ *   - the face-loop pressure assembly of the SIMPLE corrector,
+ *     physics/steady_incompressible_solver.h:8048-8181 and :8599-8652, and the
+ *     canonical Poisson assembly core/solvers/scalar_diffusion.h:39-163.
+ *
+ * Conventions
+ *   - Every entry point returns a cfdxhip_status and never throws; the reason
+ *     of the latest failure on a handle is available from *_last_error
+ *     (mirrors NativeAMGPreconditioner::last_error, hypre_amg.h:89-96).
+ *   - Pointers are host pointers unless the name ends in _device, in which
+ *     case they are device pointers on the handle's device and the call is
+ *     ordered on the handle's stream.
+ *   - CSR: uint32 row offsets [n+1], uint32 columns strictly ascending per
+ *     row, double values (core/linalg/sparse_matrix.h:27-31).
+ *   - One handle per host thread: handles are not reentrant.
+ */
+#ifndef CFDXHIP_H
+#define CFDXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cfdxhip_status {
+    CFDXHIP_OK = 0,
+    CFDXHIP_ERR_INVALID_ARGUMENT = 1,
+    CFDXHIP_ERR_NOT_READY = 2,
+    CFDXHIP_ERR_PATTERN_CHANGED = 3,
+    CFDXHIP_ERR_SETUP_FAILED = 4,
+    CFDXHIP_ERR_NUMERIC = 5,
+    CFDXHIP_ERR_DEVICE = 6,
+    CFDXHIP_ERR_OUT_OF_MEMORY = 7,
+    CFDXHIP_ERR_INTERNAL = 8
+} cfdxhip_status;
+
+/* SolverStatus, core/linalg/solver_status.h:18-23. */
+enum {
+    CFDXHIP_SOLVER_CONVERGED = 0,
+    CFDXHIP_SOLVER_MAX_ITER_REACHED = 1,
+    CFDXHIP_SOLVER_DIVERGED = 2,
+    CFDXHIP_SOLVER_NOT_APPLICABLE = 3
+};
+
+/* SolverResult (core/linalg/solver_status.h:45-72) plus solve metrics. */
+typedef struct cfdxhip_result {
+    uint32_t status;
+    uint32_t reserved0;
+    uint64_t iterations;
+    double residual;          /* absolute FP64 true residual ||b - A x||_2 */
+    double residual_relative; /* residual / ||b||_2 */
+    uint64_t residual_replacements;
+    double max_true_recursive_gap;
+    double min_true_residual;
+    double max_true_residual;
+    uint64_t first_residual_replacement;
+    uint64_t last_residual_replacement;
+    /* metrics of this build */
+    double solve_seconds;          /* wall time of the solve call */
+    uint64_t preconditioner_applications;
+    uint64_t learned_stage_applications;
+    uint64_t learned_stage_fallbacks;
+} cfdxhip_result;
+
+/* NativeAMGMethod (hypre_amg.h:19-22). */
+enum { CFDXHIP_AMG_BOOMER_STYLE = 0, CFDXHIP_AMG_SMOOTHED_AGGREGATION = 1 };
+/* AMGMemoryPolicy (hypre_amg.h:13-17). */
+enum { CFDXHIP_POLICY_LOW = 0, CFDXHIP_POLICY_BALANCED = 1, CFDXHIP_POLICY_FAST = 2 };
+/* Precision of the V-cycle (hierarchy values, smoother, transfers). */
+enum { CFDXHIP_PRECISION_FP64 = 0, CFDXHIP_PRECISION_FP32 = 1 };
+
+typedef struct cfdxhip_amg_options {
+    uint32_t method;                       /* CFDXHIP_AMG_* (default smoothed aggregation) */
+    uint32_t policy;                       /* CFDXHIP_POLICY_* (default balanced) */
+    uint32_t require_smoother_certificate; /* 1: fail setup without the omega certificate */
+    uint32_t precision;                    /* V-cycle precision for apply/FGMRES */
+    uint32_t reorder;                      /* 1: locality ordering of the hierarchy (default) */
+    uint32_t host_threads;                 /* 0: all host cores for setup */
+    int32_t device;                        /* HIP device ordinal; < 0: host-only handle
+                                              (setup + introspection, no device work) */
+    uint32_t reserved[9];
+} cfdxhip_amg_options;
+
+typedef struct cfdxhip_amg cfdxhip_amg;
+
+/* Fills defaults: smoothed aggregation, balanced policy, certificate on,
+ * FP64, reordering on, device 0. */
+void cfdxhip_amg_options_default(cfdxhip_amg_options* options);
+
+/* NativeAMGPreconditioner(...) + configure(AMGMemoryPolicy): hypre_amg.h:35-49. */
+cfdxhip_status cfdxhip_amg_create(const cfdxhip_amg_options* options, cfdxhip_amg** out);
+/* Preconditioner::setup(const SparseMatrix&): hypre_amg.cpp:163-236. Copies A. */
+cfdxhip_status cfdxhip_amg_setup(cfdxhip_amg* h, size_t n, const uint32_t* row_offsets,
+                                 const uint32_t* columns, const double* values);
+/* Preconditioner::update_values: hypre_amg.cpp:238-286. Same pattern required. */
+cfdxhip_status cfdxhip_amg_update_values(cfdxhip_amg* h, const double* values);
+/* Preconditioner::apply(r, z): hypre_amg.cpp:288-330. One V-cycle, z = M^{-1} r. */
+cfdxhip_status cfdxhip_amg_apply(cfdxhip_amg* h, const double* r, double* z);
+/* solve_cg_prepared(A, b, x, pc, max_iter, tol): cg_solver.h:653-667.
+ * x: initial guess in, solution out. */
+cfdxhip_status cfdxhip_amg_solve_cg(cfdxhip_amg* h, const double* b, double* x, size_t max_iter,
+                                    double tolerance, cfdxhip_result* result);
+/* solve_fgmres(A, b, x, restart, max_iter, tol, pc): gmres_solver.h:678-692,
+ * FP64 outer iteration; the V-cycle runs in options.precision. */
+cfdxhip_status cfdxhip_amg_solve_fgmres(cfdxhip_amg* h, const double* b, double* x, int restart,
+                                        size_t max_iter, double tolerance, cfdxhip_result* result);
+/* Device-resident variants (b, x on the device, caller ordering). */
+cfdxhip_status cfdxhip_amg_solve_cg_device(cfdxhip_amg* h, const double* b, double* x, size_t max_iter,
+                                           double tolerance, cfdxhip_result* result);
+cfdxhip_status cfdxhip_amg_solve_fgmres_device(cfdxhip_amg* h, const double* b, double* x, int restart,
+                                               size_t max_iter, double tolerance, cfdxhip_result* result);
+/* NativeAMGPreconditioner::last_error: hypre_amg.h:89-96. Never NULL. */
+const char* cfdxhip_amg_last_error(const cfdxhip_amg* h);
+void cfdxhip_amg_destroy(cfdxhip_amg* h);
+
+/* Hierarchy introspection (hierarchy_level_sizes / _nnz, hypre_amg.h:57-66,
+ * and the first aggregation, amg_preconditioner.h:349-355). Matrices and
+ * transfers are returned in the caller's (reference) ordering. */
+cfdxhip_status cfdxhip_amg_num_levels(const cfdxhip_amg* h, size_t* levels);
+cfdxhip_status cfdxhip_amg_level_info(const cfdxhip_amg* h, size_t level, size_t* n, size_t* nnz,
+                                      size_t* prolongation_nnz);
+cfdxhip_status cfdxhip_amg_level_matrix(const cfdxhip_amg* h, size_t level, uint32_t* row_offsets,
+                                        uint32_t* columns, double* values);
+cfdxhip_status cfdxhip_amg_level_transfer(const cfdxhip_amg* h, size_t level, uint64_t* aggregate,
+                                          uint32_t* p_row_offsets, uint32_t* p_columns, double* p_values);
+cfdxhip_status cfdxhip_amg_smoother_omega(const cfdxhip_amg* h, double* omega);
+/* Seconds spent in the last setup: host construction, device upload. */
+cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* host_seconds, double* device_seconds);
+
+/* Plain CSR SpMV on the device, y = A x, FP64 (sparse_matrix.h:226-242).
+ * Kept for measurement: runs the hierarchy's finest-level kernel on the
+ * caller-ordered vectors. repeat >= 1 launches back to back; the kernel time
+ * of the last batch (HIP events on the handle's stream) is returned. */
+cfdxhip_status cfdxhip_amg_spmv_device(cfdxhip_amg* h, uint32_t precision, const void* x, void* y,
+                                       uint32_t repeat, double* seconds_per_launch);
+
+/* ------------------------------------------------------------ meshes ---- */
+
+/* Synthetic unstructured hex meshes (inputs of the workloads). */
+enum { CFDXHIP_MESH_BOX = 0, CFDXHIP_MESH_EXTRUDED_2D = 1, CFDXHIP_MESH_CHANNEL_STEP = 2 };
+typedef struct cfdxhip_mesh_spec {
+    uint32_t kind;
+    uint32_t nx, ny, nz;
+    double lx, ly, lz;
+    double jitter;
+    uint32_t wall_layers;
+    uint32_t permute;
+    double first_aspect;
+    double growth;
+    double step_height_fraction;
+    double step_length_fraction;
+    uint64_t seed;
+} cfdxhip_mesh_spec;
+
+typedef struct cfdxhip_host_mesh cfdxhip_host_mesh;
+typedef struct cfdxhip_mesh_sizes {
+    uint64_t n_points, n_faces, n_internal_faces, n_cells, n_face_vertices, n_cell_faces, n_patches;
+} cfdxhip_mesh_sizes;
+
+cfdxhip_status cfdxhip_mesh_generate(const cfdxhip_mesh_spec* spec, cfdxhip_host_mesh** out);
+cfdxhip_status cfdxhip_mesh_sizes_get(const cfdxhip_host_mesh* m, cfdxhip_mesh_sizes* sizes);
+/* Copies the mesh arrays; any pointer may be NULL to skip that array.
+ * patch_kind: 0 wall, 1 empty, 2 symmetry (PatchType, mesh/boundary.h:31-40).
+ * patch_of_face: patch id of boundary faces, UINT32_MAX for internal faces.
+ * cell_ijk: lattice coordinates of each cell (3 per cell). */
+cfdxhip_status cfdxhip_mesh_copy(const cfdxhip_host_mesh* m, double* px, double* py, double* pz,
+                                 uint32_t* face_offsets, uint32_t* face_vertices, uint32_t* owner,
+                                 int32_t* neighbour, uint32_t* cell_offsets, uint32_t* cell_faces,
+                                 uint32_t* patch_of_face, uint8_t* patch_kind, uint32_t* cell_ijk);
+void cfdxhip_mesh_destroy(cfdxhip_host_mesh* m);
+
+/* ------------------------------------------- finite-volume assembly ---- */
+
+/* Device-resident mesh: points, face->vertex CSR, owner/neighbour (neighbour
+ * -1 on boundary faces), cell->face CSR, patch id per face (UINT32_MAX for
+ * internal faces) and patch kinds (0 wall, 1 empty, 2 symmetry). Geometry is
+ * computed on the device at creation: build_fv_geometry,
+ * physics/finite_volume_transport.h:92-175. */
+typedef struct cfdxhip_fv cfdxhip_fv;
+cfdxhip_status cfdxhip_fv_create(int32_t device, size_t n_points, const double* px, const double* py,
+                                 const double* pz, size_t n_faces, const uint32_t* face_offsets,
+                                 const uint32_t* face_vertices, const uint32_t* owner, const int32_t* neighbour,
+                                 size_t n_cells, const uint32_t* cell_offsets, const uint32_t* cell_faces,
+                                 size_t n_patches, const uint32_t* patch_of_face, const uint8_t* patch_kind,
+                                 cfdxhip_fv** out);
+void cfdxhip_fv_destroy(cfdxhip_fv* fv);
+const char* cfdxhip_fv_last_error(const cfdxhip_fv* fv);
+/* Geometry copy-out: xyz triplets per face / cell, volumes. NULL skips. */
+cfdxhip_status cfdxhip_fv_geometry(cfdxhip_fv* fv, double* face_centres, double* face_area_vectors,
+                                   double* cell_centres, double* cell_volumes);
+/* gauss_gradient_with_boundary, steady_incompressible_solver.h:1634-1750.
+ * bc_type per patch: 0 zero-gradient, 1 fixed value, 2 fixed gradient.
+ * grad: 3 x n_cells, component-major. */
+cfdxhip_status cfdxhip_fv_gauss_gradient(cfdxhip_fv* fv, const uint8_t* bc_type, const double* bc_value,
+                                         const double* p, double* grad);
+/* make_mass_flux, steady_incompressible_solver.h:1842-1905. vbc_type per
+ * patch: 0 zero-gradient, 1 fixed value (3 values per patch). U: 3 x n_cells. */
+cfdxhip_status cfdxhip_fv_mass_flux(cfdxhip_fv* fv, const uint8_t* vbc_type, const double* vbc_value, double rho,
+                                    const double* U, double* phi);
+/* Sparsity of the pressure system (steady_incompressible_solver.h:8599-8652):
+ * with no fixed-pressure patch the reference cell is eliminated. row_offsets
+ * and columns may be NULL to query sizes. */
+cfdxhip_status cfdxhip_fv_pressure_pattern(cfdxhip_fv* fv, const uint8_t* pbc_type, size_t reference_cell,
+                                           size_t* n_rows, size_t* nnz, uint32_t* row_offsets, uint32_t* columns);
+/* SIMPLE corrector assembly (steady_incompressible_solver.h:8048-8181):
+ * values of the pattern above and the RHS. rAU: 3 x n_cells, phiHbyA: per
+ * face, p: per cell. Host pointers. */
+cfdxhip_status cfdxhip_fv_pressure_assemble(cfdxhip_fv* fv, const uint8_t* pbc_type, const double* pbc_value,
+                                            double rho, const double* rAU, const double* phiHbyA, const double* p,
+                                            size_t reference_cell, double* values, double* rhs);
+/* Same with device pointers (rAU, phiHbyA, p, values, rhs). */
+cfdxhip_status cfdxhip_fv_pressure_assemble_device(cfdxhip_fv* fv, const uint8_t* pbc_type,
+                                                   const double* pbc_value, double rho, const double* rAU,
+                                                   const double* phiHbyA, const double* p, size_t reference_cell,
+                                                   double* values, double* rhs);
+/* assemble_cell_diffusion_matrix (core/solvers/scalar_diffusion.h:39-163):
+ * face_value NaN = unprescribed face; face_is_neumann selects q_n faces.
+ * Pattern: diagonal + face neighbours (n_cells rows). */
+cfdxhip_status cfdxhip_fv_diffusion_assemble(cfdxhip_fv* fv, double gamma, const double* source,
+                                             const double* face_value, const uint8_t* face_is_neumann,
+                                             size_t* nnz, uint32_t* row_offsets, uint32_t* columns,
+                                             double* values, double* rhs);
+
+/* Version string of the library build. */
+const char* cfdxhip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFDXHIP_H */

@@ -1,0 +1,1646 @@
+// CFDX pressure-correction oracle: an FP64 CPU restatement of the reference
+// algorithm, written from scratch and used ONLY as test infrastructure.
+//
+// This file is the checker, never the product. Only tests/, the smoke check
+// and the CPU-baseline leg of bench.py load the shared library built from it.
+// The shipped path (libcfdxhip.so) never links, loads or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the root of the stevendaix/cfdx_dev sources). Floating-point expressions
+// keep the reference's operation order so that, compiled without FMA
+// contraction, results reproduce the reference bit-for-bit; that claim is
+// checked against oracle/_ref (the reference's own sources) in tests/.
+//
+// Pinning: every function here is compared with oracle/_ref on seeded
+// meshes/matrices, and additionally with closed forms (uniform-mesh
+// stencils, unit-cube volumes, P*1 = 1, brute-force P^T A P) in
+// tests/test_oracle_pinning.py.
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <queue>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace {
+
+using std::size_t;
+
+struct V3 {
+    double x = 0.0, y = 0.0, z = 0.0;
+};
+inline V3 add(const V3& a, const V3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 sub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 scale(const V3& a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(const V3& a, const V3& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double mag(const V3& a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// ---------------------------------------------------------------- geometry --
+
+// Triangle-fan face geometry: core/geometry/face_geometry.h:50-90.
+bool face_geometry(const double* px, const double* py, const double* pz,
+                   const uint32_t* verts, uint32_t off, uint32_t nv,
+                   V3& centre, V3& Sf) {
+    if (nv < 3) return false;
+    const uint32_t v0 = verts[off];
+    const V3 p0{px[v0], py[v0], pz[v0]};
+    V3 S{}, wc{};
+    double tri_area = 0.0;
+    for (uint32_t k = 1; k + 1 < nv; ++k) {
+        const uint32_t va = verts[off + k], vb = verts[off + k + 1];
+        const V3 pa{px[va], py[va], pz[va]};
+        const V3 pb{px[vb], py[vb], pz[vb]};
+        const V3 tS = scale(cross(sub(pa, p0), sub(pb, p0)), 0.5);
+        const double ta = mag(tS);
+        if (!std::isfinite(ta)) return false;
+        S = add(S, tS);
+        wc = add(wc, scale(add(add(p0, pa), pb), ta / 3.0));
+        tri_area += ta;
+    }
+    const double area = mag(S);
+    if (!(tri_area > 0.0) || !(area > 0.0) || !std::isfinite(area)) return false;
+    centre = scale(wc, 1.0 / tri_area);
+    Sf = S;
+    return true;
+}
+
+// Cell centre (area-weighted face-centre mean) and pyramid volume:
+// core/geometry/cell_geometry.h:49-113 (3-D branch) with the oriented
+// wrapper :116-147.
+bool cell_geometry(const std::vector<V3>& fc, const std::vector<V3>& fs,
+                   const uint32_t* cf, size_t nfc, uint32_t cell,
+                   const uint32_t* owner, V3& centre, double& volume) {
+    if (nfc == 0) return false;
+    V3 ws{};
+    double total = 0.0;
+    for (size_t k = 0; k < nfc; ++k) {
+        const uint32_t f = cf[k];
+        const V3 s = owner[f] == cell ? fs[f] : scale(fs[f], -1.0);
+        const double a = mag(s);
+        ws = add(ws, scale(fc[f], a));
+        total += a;
+    }
+    centre = total > 0.0 ? scale(ws, 1.0 / total) : V3{};
+    double sv = 0.0;
+    for (size_t k = 0; k < nfc; ++k) {
+        const uint32_t f = cf[k];
+        const V3 s = owner[f] == cell ? fs[f] : scale(fs[f], -1.0);
+        if (!(mag(s) > 0.0)) return false;
+        sv += dot(sub(fc[f], centre), s);
+    }
+    sv /= 3.0;
+    volume = std::abs(sv);
+    return true;
+}
+
+struct Geometry {
+    std::vector<V3> face_centre, face_sf, cell_centre;
+    std::vector<double> cell_volume;
+};
+
+// build_fv_geometry: physics/finite_volume_transport.h:92-175 (face pass,
+// provisional cell pass, owner->neighbour orientation :128-149 via
+// ensure_face_orientation core/geometry/face_geometry.h:115-127, final pass).
+bool build_geometry(size_t np, const double* px, const double* py, const double* pz,
+                    size_t nf, const uint32_t* foff, const uint32_t* fvtx,
+                    const uint32_t* owner, const int32_t* neigh,
+                    size_t nc, const uint32_t* coff, const uint32_t* cfaces,
+                    Geometry& g) {
+    (void)np;
+    g.face_centre.assign(nf, {});
+    g.face_sf.assign(nf, {});
+    g.cell_centre.assign(nc, {});
+    g.cell_volume.assign(nc, 0.0);
+    for (size_t f = 0; f < nf; ++f)
+        if (!face_geometry(px, py, pz, fvtx, foff[f], foff[f + 1] - foff[f],
+                           g.face_centre[f], g.face_sf[f]))
+            return false;
+    for (size_t c = 0; c < nc; ++c)
+        if (!cell_geometry(g.face_centre, g.face_sf, cfaces + coff[c], coff[c + 1] - coff[c],
+                           uint32_t(c), owner, g.cell_centre[c], g.cell_volume[c]))
+            return false;
+    for (size_t f = 0; f < nf; ++f) {
+        const uint32_t o = owner[f];
+        V3 d = neigh[f] >= 0 ? sub(g.cell_centre[size_t(neigh[f])], g.cell_centre[o])
+                             : sub(g.face_centre[f], g.cell_centre[o]);
+        if (dot(g.face_sf[f], d) < 0.0) g.face_sf[f] = scale(g.face_sf[f], -1.0);
+    }
+    for (size_t c = 0; c < nc; ++c) {
+        if (!cell_geometry(g.face_centre, g.face_sf, cfaces + coff[c], coff[c + 1] - coff[c],
+                           uint32_t(c), owner, g.cell_centre[c], g.cell_volume[c]))
+            return false;
+        if (!(g.cell_volume[c] > 0.0)) return false;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------- CSR --
+
+struct Csr {
+    size_t n = 0;
+    std::vector<uint32_t> rowp, cols;
+    std::vector<double> vals;
+    size_t nnz() const { return vals.size(); }
+};
+
+// SparseMatrix::push_back + finalize: core/linalg/sparse_matrix.h:54-131.
+// Stable sort by (row, col); duplicates summed in long double.
+struct Coo {
+    size_t n = 0;
+    std::vector<size_t> r;
+    std::vector<uint32_t> c;
+    std::vector<double> v;
+    void push(size_t row, size_t col, double val) {
+        r.push_back(row);
+        c.push_back(uint32_t(col));
+        v.push_back(val);
+    }
+    bool finalize(Csr& out) const {
+        std::vector<size_t> order(v.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            if (r[a] != r[b]) return r[a] < r[b];
+            return c[a] < c[b];
+        });
+        out.n = n;
+        out.rowp.assign(n + 1, 0);
+        out.cols.clear();
+        out.vals.clear();
+        size_t prow = n;
+        uint32_t pcol = 0;
+        long double acc = 0.0L;
+        auto flush = [&]() {
+            if (prow == n) return true;
+            if (!std::isfinite((double)acc)) return false;
+            out.vals.push_back(double(acc));
+            out.cols.push_back(pcol);
+            ++out.rowp[prow + 1];
+            return true;
+        };
+        for (size_t k : order) {
+            if (r[k] != prow || c[k] != pcol) {
+                if (!flush()) return false;
+                prow = r[k];
+                pcol = c[k];
+                acc = 0.0L;
+            }
+            acc += (long double)v[k];
+        }
+        if (!flush()) return false;
+        for (size_t i = 0; i < n; ++i) out.rowp[i + 1] += out.rowp[i];
+        return true;
+    }
+};
+
+void matvec(const Csr& A, const double* x, double* y) {
+    for (size_t i = 0; i < A.n; ++i) {
+        double s = 0.0;
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) s += A.vals[k] * x[A.cols[k]];
+        y[i] = s;
+    }
+}
+
+// krylov_norm2 (scaled sum of squares): core/linalg/krylov_reductions.h:84-115.
+double knorm2(const double* a, size_t n) {
+    double scale_ = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(a[i])) return std::numeric_limits<double>::quiet_NaN();
+        scale_ = std::max(scale_, std::abs(a[i]));
+    }
+    if (scale_ == 0.0) return 0.0;
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double q = a[i] / scale_;
+        s += q * q;
+    }
+    if (!std::isfinite(s)) return std::numeric_limits<double>::quiet_NaN();
+    return scale_ * std::sqrt(std::max(0.0, s));
+}
+
+// krylov_dot_view: core/linalg/krylov_reductions.h:63-74.
+double kdot(const double* a, const double* b, size_t n) {
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+// ---------------------------------------------------------------- AMG -----
+
+using TransferRow = std::vector<std::pair<size_t, double>>;
+
+struct Level {
+    Csr A;
+    std::vector<double> inv_diag;
+    std::vector<size_t> aggregate;
+    std::vector<TransferRow> P;
+    // Galerkin symbolic plan (galerkin_csr_product.h:33-148).
+    std::vector<uint32_t> ap_off, ap_col, c_off, c_col, t_off, t_row, t_pos;
+};
+
+// build_diagonal: core/linalg/amg_preconditioner.h:1401-1421.
+bool build_diagonal(Level& L) {
+    const size_t n = L.A.n;
+    L.inv_diag.assign(n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        double d = 0.0;
+        for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k)
+            if (L.A.cols[k] == i) d += L.A.vals[k];
+        if (!std::isfinite(d) || std::abs(d) <= 1e-30) return false;
+        L.inv_diag[i] = 1.0 / d;
+        if (!std::isfinite(L.inv_diag[i])) return false;
+    }
+    return true;
+}
+
+// build_strength_graph: core/linalg/amg_preconditioner.h:1466-1509.
+std::vector<std::vector<size_t>> strength_graph(const Level& L, double theta) {
+    const size_t n = L.A.n;
+    std::vector<std::vector<size_t>> S(n);
+    const double max_row_sum = 0.9;
+    for (size_t i = 0; i < n; ++i) {
+        const double di = 1.0 / L.inv_diag[i];
+        double row_sum = di, opp = 0.0, absmax = 0.0;
+        for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k) {
+            if (L.A.cols[k] == i) continue;
+            const double v = L.A.vals[k];
+            row_sum += v;
+            absmax = std::max(absmax, std::abs(v));
+            if (di * v < 0.0) opp = std::max(opp, std::abs(v));
+        }
+        if (std::abs(row_sum) > std::abs(di) * max_row_sum) continue;
+        const bool use_opp = opp > 0.0;
+        const double sc = use_opp ? opp : absmax;
+        if (!(sc > 0.0) || !std::isfinite(sc)) continue;
+        for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k) {
+            const size_t j = L.A.cols[k];
+            const double v = L.A.vals[k];
+            if (j == i || j >= n) continue;
+            if (use_opp && di * v >= 0.0) continue;
+            if (std::abs(v) + std::numeric_limits<double>::epsilon() < theta * sc) continue;
+            S[i].push_back(j);
+        }
+        std::sort(S[i].begin(), S[i].end());
+        S[i].erase(std::unique(S[i].begin(), S[i].end()), S[i].end());
+    }
+    return S;
+}
+
+// smooth_tentative_prolongator: core/linalg/amg_preconditioner.h:1968-2044.
+void smooth_tentative(const Level& L, const std::vector<size_t>& agg, size_t nc,
+                      std::vector<TransferRow>& P, double filter) {
+    const size_t n = L.A.n;
+    const double w = 2.0 / 3.0;
+    P.assign(n, {});
+    std::vector<size_t> stamp(nc, n), pos(nc, 0);
+    std::vector<std::pair<size_t, double>> wts;
+    for (size_t i = 0; i < n; ++i) {
+        wts.clear();
+        auto acc = [&](size_t c, double v) {
+            if (stamp[c] != i) {
+                stamp[c] = i;
+                pos[c] = wts.size();
+                wts.emplace_back(c, 0.0);
+            }
+            wts[pos[c]].second += v;
+        };
+        acc(agg[i], 1.0);
+        for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k)
+            acc(agg[L.A.cols[k]], -w * L.inv_diag[i] * L.A.vals[k]);
+        std::sort(wts.begin(), wts.end(),
+                  [](const auto& a, const auto& b) { return a.first < b.first; });
+        double sum = 0.0;
+        for (const auto& e : wts)
+            if (std::isfinite(e.second) && std::abs(e.second) > 1e-14) sum += e.second;
+        if (!std::isfinite(sum) || std::abs(sum) <= 1e-14) {
+            P[i].push_back({agg[i], 1.0});
+            continue;
+        }
+        double largest = 0.0;
+        for (const auto& e : wts) largest = std::max(largest, std::abs(e.second / sum));
+        for (const auto& e : wts) {
+            const double nv = e.second / sum;
+            if (std::isfinite(nv) && std::abs(nv) > 1e-14 && std::abs(nv) >= filter * largest)
+                P[i].push_back({e.first, nv});
+        }
+        double kept = 0.0;
+        for (const auto& e : P[i]) kept += e.second;
+        if (std::isfinite(kept) && std::abs(kept) > 1e-14) {
+            for (auto& e : P[i]) e.second /= kept;
+        } else {
+            P[i].clear();
+        }
+        if (P[i].empty()) P[i].push_back({agg[i], 1.0});
+    }
+}
+
+// build_smoothed_aggregation_interpolation:
+// core/linalg/amg_preconditioner.h:1887-1964 (symmetrised strength graph,
+// lazy max-heap of unassigned degree with lowest-index tie break).
+void sa_interpolation(const Level& L, double theta, std::vector<TransferRow>& P,
+                      std::vector<size_t>& agg, size_t& nc, double filter) {
+    const size_t n = L.A.n;
+    const auto S = strength_graph(L, theta);
+    std::vector<std::vector<size_t>> nb = S;
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j : S[i]) nb[j].push_back(i);
+    for (auto& v : nb) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    agg.assign(n, n);
+    nc = 0;
+    size_t remaining = n;
+    std::vector<size_t> deg(n, 0);
+    std::priority_queue<std::pair<size_t, size_t>> heap;
+    for (size_t i = 0; i < n; ++i) {
+        deg[i] = nb[i].size();
+        heap.emplace(deg[i], n - i);
+    }
+    auto assign = [&](size_t v, size_t id) {
+        if (agg[v] != n) return;
+        agg[v] = id;
+        --remaining;
+        for (size_t u : nb[v])
+            if (agg[u] == n) {
+                --deg[u];
+                heap.emplace(deg[u], n - u);
+            }
+    };
+    while (remaining > 0) {
+        size_t seed = n;
+        while (!heap.empty()) {
+            const auto top = heap.top();
+            heap.pop();
+            const size_t idx = n - top.second;
+            if (agg[idx] != n || deg[idx] != top.first) continue;
+            seed = idx;
+            break;
+        }
+        if (seed == n) return;  // unreachable for a consistent heap
+        const size_t id = nc++;
+        assign(seed, id);
+        for (size_t j : nb[seed]) assign(j, id);
+    }
+    smooth_tentative(L, agg, nc, P, filter);
+}
+
+// build_cf_interpolation (Ruge-Stueben first pass with dynamic influence,
+// second-pass F-F repair, promotion, classical weights row-normalised):
+// core/linalg/amg_preconditioner.h:1511-1885. The second pass restarts its
+// scan after each promotion in the reference (:1617-1645); promotions only
+// add C points, which cannot create a violation at an earlier index, so the
+// single ordered scan below visits the same points in the same order.
+bool cf_interpolation(const Level& L, double theta, std::vector<TransferRow>& P,
+                      std::vector<size_t>& agg, size_t& nc) {
+    const size_t n = L.A.n;
+    const auto S = strength_graph(L, theta);
+    std::vector<std::vector<size_t>> T(n);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j : S[i])
+            if (j < n) T[j].push_back(i);
+    enum : unsigned char { U = 0, F = 1, C = 2 };
+    std::vector<unsigned char> pt(n, U);
+    std::vector<size_t> infl(n, 0);
+    std::priority_queue<std::pair<size_t, size_t>> heap;
+    for (size_t i = 0; i < n; ++i) {
+        infl[i] = T[i].size();
+        heap.emplace(infl[i], n - i);
+    }
+    size_t undecided = n;
+    while (undecided > 0) {
+        size_t best = n;
+        while (!heap.empty()) {
+            const auto top = heap.top();
+            heap.pop();
+            const size_t i = n - top.second;
+            if (pt[i] == U && top.first == infl[i]) {
+                best = i;
+                break;
+            }
+        }
+        if (best == n) return false;
+        pt[best] = C;
+        --undecided;
+        for (size_t j : T[best]) {
+            if (pt[j] == U) {
+                pt[j] = F;
+                --undecided;
+                for (size_t k : S[j])
+                    if (pt[k] == U) {
+                        ++infl[k];
+                        heap.emplace(infl[k], n - k);
+                    }
+            }
+        }
+    }
+    auto common_c = [&](size_t i, size_t j) {
+        for (size_t ci : S[i]) {
+            if (pt[ci] != C) continue;
+            for (size_t cj : S[j])
+                if (ci == cj && pt[cj] == C) return true;
+        }
+        return false;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        if (pt[i] != F) continue;
+        for (size_t j : S[i]) {
+            if (pt[j] != F) continue;
+            if (common_c(i, j)) continue;
+            pt[i] = C;
+            break;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (pt[i] != F) continue;
+        bool has = false;
+        for (size_t j : S[i])
+            if (pt[j] == C) {
+                has = true;
+                break;
+            }
+        if (!has) pt[i] = C;
+    }
+    std::vector<size_t> cidx(n, n);
+    nc = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (pt[i] == C) cidx[i] = nc++;
+    auto aval = [&](size_t i, size_t j) {
+        double v = 0.0;
+        for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k)
+            if (L.A.cols[k] == j) v += L.A.vals[k];
+        return v;
+    };
+    P.assign(n, {});
+    agg.assign(n, 0);
+    std::vector<size_t> stamp(nc, n), pos(nc, 0);
+    std::vector<std::pair<size_t, double>> wts;
+    auto accw = [&](size_t row, size_t c, double v) {
+        if (stamp[c] != row) {
+            stamp[c] = row;
+            pos[c] = wts.size();
+            wts.emplace_back(c, 0.0);
+        }
+        wts[pos[c]].second += v;
+    };
+    auto fail = [&]() {
+        P.clear();
+        agg.clear();
+        nc = 0;
+        return false;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        if (pt[i] == C) {
+            P[i].push_back({cidx[i], 1.0});
+            agg[i] = cidx[i];
+            continue;
+        }
+        const double diag = 1.0 / L.inv_diag[i];
+        std::vector<size_t> cn;
+        for (size_t j : S[i])
+            if (pt[j] == C) cn.push_back(j);
+        if (cn.empty()) return fail();
+        double weak = 0.0;
+        for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k) {
+            const size_t j = L.A.cols[k];
+            if (j == i) continue;
+            if (!std::binary_search(S[i].begin(), S[i].end(), j)) weak += L.A.vals[k];
+        }
+        const double den = diag + weak;
+        if (!std::isfinite(den) || std::abs(den) <= 1e-30) return fail();
+        wts.clear();
+        for (size_t j : cn) accw(i, cidx[j], aval(i, j));
+        for (size_t k : S[i]) {
+            if (pt[k] != F) continue;
+            double csum = 0.0;
+            std::vector<size_t> kc;
+            for (size_t m : S[k])
+                if (pt[m] == C) {
+                    csum += aval(k, m);
+                    kc.push_back(m);
+                }
+            if (!std::isfinite(csum) || std::abs(csum) <= 1e-30) return fail();
+            const double aik = aval(i, k);
+            for (size_t j : kc) {
+                const double akj = aval(k, j);
+                accw(i, cidx[j], aik * akj / csum);
+            }
+        }
+        std::sort(wts.begin(), wts.end(),
+                  [](const auto& a, const auto& b) { return a.first < b.first; });
+        double largest = -1.0;
+        for (const auto& e : wts) {
+            const double w = -e.second / den;
+            if (!std::isfinite(w)) return fail();
+            P[i].push_back({e.first, w});
+            if (std::abs(w) > largest) {
+                largest = std::abs(w);
+                agg[i] = e.first;
+            }
+        }
+        if (P[i].empty()) return fail();
+        double rs = 0.0;
+        for (const auto& e : P[i]) rs += e.second;
+        if (!std::isfinite(rs) || std::abs(rs) <= 1e-30) return fail();
+        for (auto& e : P[i]) {
+            e.second /= rs;
+            if (!std::isfinite(e.second)) return fail();
+        }
+    }
+    return true;
+}
+
+// GalerkinCSRProduct::symbolic: core/linalg/galerkin_csr_product.h:33-148.
+void galerkin_symbolic(Level& L, size_t nc) {
+    const size_t n = L.A.n;
+    std::vector<uint32_t> cnt(nc, 0);
+    for (size_t i = 0; i < n; ++i)
+        for (const auto& e : L.P[i]) ++cnt[e.first];
+    L.t_off.assign(nc + 1, 0);
+    for (size_t c = 0; c < nc; ++c) L.t_off[c + 1] = L.t_off[c] + cnt[c];
+    L.t_row.assign(L.t_off[nc], 0);
+    L.t_pos.assign(L.t_off[nc], 0);
+    std::vector<uint32_t> cur(L.t_off.begin(), L.t_off.end() - 1);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < L.P[i].size(); ++j) {
+            const size_t c = L.P[i][j].first;
+            L.t_row[cur[c]] = uint32_t(i);
+            L.t_pos[cur[c]] = uint32_t(j);
+            ++cur[c];
+        }
+    std::vector<size_t> seen(nc, std::numeric_limits<size_t>::max());
+    std::vector<uint32_t> scratch;
+    L.ap_off.assign(1, 0);
+    L.ap_col.clear();
+    for (size_t i = 0; i < n; ++i) {
+        scratch.clear();
+        for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k)
+            for (const auto& e : L.P[L.A.cols[k]])
+                if (seen[e.first] != i) {
+                    seen[e.first] = i;
+                    scratch.push_back(uint32_t(e.first));
+                }
+        std::sort(scratch.begin(), scratch.end());
+        L.ap_col.insert(L.ap_col.end(), scratch.begin(), scratch.end());
+        L.ap_off.push_back(uint32_t(L.ap_col.size()));
+    }
+    std::fill(seen.begin(), seen.end(), std::numeric_limits<size_t>::max());
+    L.c_off.assign(1, 0);
+    L.c_col.clear();
+    for (size_t c = 0; c < nc; ++c) {
+        scratch.clear();
+        for (uint32_t t = L.t_off[c]; t < L.t_off[c + 1]; ++t) {
+            const size_t i = L.t_row[t];
+            for (uint32_t p = L.ap_off[i]; p < L.ap_off[i + 1]; ++p) {
+                const uint32_t cj = L.ap_col[p];
+                if (seen[cj] != c) {
+                    seen[cj] = c;
+                    scratch.push_back(cj);
+                }
+            }
+        }
+        std::sort(scratch.begin(), scratch.end());
+        L.c_col.insert(L.c_col.end(), scratch.begin(), scratch.end());
+        L.c_off.push_back(uint32_t(L.c_col.size()));
+    }
+}
+
+// GalerkinCSRProduct::accumulate (numeric): galerkin_csr_product.h:231-303,
+// then the 1e-30 pruning of numeric() :153-178.
+bool galerkin_numeric(const Level& L, size_t nc, const Csr& A, Csr& out, bool require_pattern) {
+    const size_t n = L.A.n;
+    const size_t missing = std::numeric_limits<size_t>::max();
+    std::vector<size_t> lookup(nc, missing);
+    std::vector<double> apv(L.ap_col.size(), 0.0), cv(L.c_col.size(), 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        for (uint32_t p = L.ap_off[i]; p < L.ap_off[i + 1]; ++p) lookup[L.ap_col[p]] = p;
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k)
+            for (const auto& e : L.P[A.cols[k]]) {
+                const size_t pos = lookup[e.first];
+                apv[pos] += A.vals[k] * e.second;
+                if (!std::isfinite(apv[pos])) return false;
+            }
+        for (uint32_t p = L.ap_off[i]; p < L.ap_off[i + 1]; ++p) lookup[L.ap_col[p]] = missing;
+    }
+    for (size_t c = 0; c < nc; ++c) {
+        for (uint32_t p = L.c_off[c]; p < L.c_off[c + 1]; ++p) lookup[L.c_col[p]] = p;
+        for (uint32_t t = L.t_off[c]; t < L.t_off[c + 1]; ++t) {
+            const size_t i = L.t_row[t];
+            const double w = L.P[i][L.t_pos[t]].second;
+            for (uint32_t p = L.ap_off[i]; p < L.ap_off[i + 1]; ++p) {
+                const size_t pos = lookup[L.ap_col[p]];
+                cv[pos] += w * apv[p];
+                if (!std::isfinite(cv[pos])) return false;
+            }
+        }
+        for (uint32_t p = L.c_off[c]; p < L.c_off[c + 1]; ++p) lookup[L.c_col[p]] = missing;
+    }
+    if (require_pattern) {
+        // numeric_into_existing: galerkin_csr_product.h:184-212.
+        size_t k = 0;
+        for (size_t c = 0; c < nc; ++c) {
+            size_t cursor = out.rowp[c];
+            for (uint32_t p = L.c_off[c]; p < L.c_off[c + 1]; ++p) {
+                if (std::abs(cv[p]) > 1e-30) {
+                    if (cursor == out.rowp[c + 1] || out.cols[cursor] != L.c_col[p]) return false;
+                    ++cursor;
+                }
+            }
+            if (cursor != out.rowp[c + 1]) return false;
+        }
+        for (double v : cv)
+            if (std::abs(v) > 1e-30) out.vals[k++] = v;
+        return true;
+    }
+    out.n = nc;
+    out.rowp.assign(1, 0);
+    out.cols.clear();
+    out.vals.clear();
+    for (size_t c = 0; c < nc; ++c) {
+        for (uint32_t p = L.c_off[c]; p < L.c_off[c + 1]; ++p)
+            if (std::abs(cv[p]) > 1e-30) {
+                out.cols.push_back(L.c_col[p]);
+                out.vals.push_back(cv[p]);
+            }
+        out.rowp.push_back(uint32_t(out.cols.size()));
+    }
+    return true;
+}
+
+// normalized_gershgorin_upper_bound + compute_stable_smoother_omega:
+// core/linalg/amg_preconditioner.h:1324-1399.
+bool stable_omega(const std::vector<Level>& lv, double cfg, double& omega) {
+    double mb = 0.0;
+    for (const auto& L : lv) {
+        const size_t n = L.A.n;
+        std::vector<double> d(n);
+        for (size_t i = 0; i < n; ++i) {
+            if (!(L.inv_diag[i] > 0.0) || !std::isfinite(L.inv_diag[i])) return false;
+            d[i] = 1.0 / L.inv_diag[i];
+            if (!(d[i] > 0.0) || !std::isfinite(d[i])) return false;
+        }
+        double ub = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            double rs = 0.0;
+            for (uint32_t k = L.A.rowp[i]; k < L.A.rowp[i + 1]; ++k) {
+                const size_t j = L.A.cols[k];
+                const double sc = std::sqrt(d[i]) * std::sqrt(d[j]);
+                if (!(sc > 0.0) || !std::isfinite(sc)) return false;
+                rs += std::abs(L.A.vals[k]) / sc;
+            }
+            if (!std::isfinite(rs)) return false;
+            ub = std::max(ub, rs);
+        }
+        if (!std::isfinite(ub) || !(ub > 0.0)) return false;
+        mb = std::max(mb, ub);
+    }
+    omega = std::min(cfg, 1.8 / mb);
+    return std::isfinite(omega) && omega > 0.0 && omega < 2.0;
+}
+
+struct Amg {
+    int method = 1;  // 0 = Boomer-style direct C/F, 1 = smoothed aggregation
+    double filter = 0.0;  // prolongator filter (amg_preconditioner.h:2020-2033)
+    double cfg_omega = 0.7, omega = 0.7;
+    size_t pre = 6, post = 6, max_levels = 25;
+    double theta = 0.25;
+    bool require_cert = true;
+    std::vector<Level> lv;
+    std::string error;
+};
+
+bool csr_valid(const Csr& A) {
+    if (A.n == 0 || A.rowp.size() != A.n + 1 || A.rowp[0] != 0 || A.rowp[A.n] != A.vals.size())
+        return false;
+    for (size_t i = 0; i < A.n; ++i)
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) {
+            if (A.cols[k] >= A.n || !std::isfinite(A.vals[k])) return false;
+            if (k > A.rowp[i] && A.cols[k] <= A.cols[k - 1]) return false;
+        }
+    return true;
+}
+
+// MatrixFreeVcyclePreconditioner::setup: core/linalg/amg_preconditioner.h:76-243
+// with the NativeAMGPreconditioner defaults of core/linalg/hypre_amg.cpp:163-236.
+bool amg_setup(Amg& h, const Csr& A) {
+    h.lv.clear();
+    if (!csr_valid(A)) {
+        h.error = "invalid finest matrix";
+        return false;
+    }
+    Level fine;
+    fine.A = A;
+    if (!build_diagonal(fine)) {
+        h.error = "invalid or zero diagonal on finest level";
+        return false;
+    }
+    h.lv.push_back(std::move(fine));
+    const size_t coarse_limit = 16;
+    while (h.lv.back().A.n > coarse_limit && h.lv.size() < h.max_levels) {
+        Level& L = h.lv.back();
+        std::vector<TransferRow> P;
+        std::vector<size_t> agg;
+        size_t nc = 0;
+        if (h.method == 1) {
+            sa_interpolation(L, h.theta, P, agg, nc, h.filter);
+        } else if (!cf_interpolation(L, h.theta, P, agg, nc)) {
+            h.error = "Direct-CF interpolation construction failed";
+            h.lv.clear();
+            return false;
+        }
+        if (nc == 0 || nc >= L.A.n) break;
+        L.P = std::move(P);
+        L.aggregate = std::move(agg);
+        galerkin_symbolic(L, nc);
+        Level next;
+        if (!galerkin_numeric(L, nc, L.A, next.A, false) || !csr_valid(next.A)) {
+            h.error = "invalid Galerkin coarse matrix";
+            h.lv.clear();
+            return false;
+        }
+        if (!build_diagonal(next)) {
+            h.error = "invalid or zero diagonal on coarse level";
+            h.lv.clear();
+            return false;
+        }
+        h.lv.push_back(std::move(next));
+    }
+    // The level at which coarsening stopped keeps no transfer operators.
+    h.lv.back().P.clear();
+    h.lv.back().aggregate.clear();
+    double w = 0.0;
+    if (stable_omega(h.lv, h.cfg_omega, w)) {
+        h.omega = w;
+    } else if (h.require_cert) {
+        h.error = "Jacobi stability certificate failed";
+        h.lv.clear();
+        return false;
+    } else {
+        h.omega = h.cfg_omega;
+    }
+    return true;
+}
+
+// update_values: core/linalg/amg_preconditioner.h:245-314.
+bool amg_update(Amg& h, const double* vals) {
+    if (h.lv.empty()) return false;
+    std::vector<Level> staged(h.lv.size());
+    staged[0].A = h.lv[0].A;
+    std::copy(vals, vals + staged[0].A.vals.size(), staged[0].A.vals.begin());
+    if (!csr_valid(staged[0].A) || !build_diagonal(staged[0])) return false;
+    for (size_t l = 0; l + 1 < h.lv.size(); ++l) {
+        const size_t nc = h.lv[l + 1].A.n;
+        staged[l + 1].A = h.lv[l + 1].A;
+        if (!galerkin_numeric(h.lv[l], nc, staged[l].A, staged[l + 1].A, true)) return false;
+        if (!build_diagonal(staged[l + 1])) return false;
+    }
+    // The certificate reads only A and inv_diag.
+    double w = 0.0;
+    if (!stable_omega(staged, h.cfg_omega, w)) {
+        if (h.require_cert) return false;
+        w = h.omega;
+    }
+    for (size_t l = 0; l < h.lv.size(); ++l) {
+        h.lv[l].A.vals = std::move(staged[l].A.vals);
+        h.lv[l].inv_diag = std::move(staged[l].inv_diag);
+    }
+    h.omega = w;
+    return true;
+}
+
+// smooth (damped Jacobi branch): core/linalg/amg_preconditioner.h:2369-2409.
+bool smooth(const Amg& h, size_t l, const std::vector<double>& r, std::vector<double>& x,
+            size_t sweeps) {
+    const Level& L = h.lv[l];
+    std::vector<double> Ax(r.size());
+    for (size_t s = 0; s < sweeps; ++s) {
+        matvec(L.A, x.data(), Ax.data());
+        for (size_t i = 0; i < r.size(); ++i) {
+            const double delta = L.inv_diag[i] * (r[i] - Ax[i]);
+            x[i] += h.omega * delta;
+            if (!std::isfinite(x[i])) return false;
+        }
+    }
+    return true;
+}
+
+// smooth_coarsest: core/linalg/amg_preconditioner.h:2564-2666 (dense
+// partial-pivot LU at <= 16 unknowns, else 24 Jacobi sweeps).
+bool smooth_coarsest(const Amg& h, size_t l, const std::vector<double>& r, std::vector<double>& x) {
+    const Csr& A = h.lv[l].A;
+    const size_t n = A.n;
+    if (n > 16) {
+        std::fill(x.begin(), x.end(), 0.0);
+        return smooth(h, l, r, x, 24);
+    }
+    std::vector<double> m(n * n, 0.0), b(n);
+    for (size_t i = 0; i < n; ++i) {
+        b[i] = r[i];
+        for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) m[i * n + A.cols[k]] += A.vals[k];
+    }
+    double ms = 0.0;
+    for (double v : m) ms = std::max(ms, std::abs(v));
+    const double tol = 128.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, ms);
+    for (size_t k = 0; k < n; ++k) {
+        size_t piv = k;
+        double pa = std::abs(m[k * n + k]);
+        for (size_t i = k + 1; i < n; ++i) {
+            const double c = std::abs(m[i * n + k]);
+            if (c > pa) {
+                pa = c;
+                piv = i;
+            }
+        }
+        if (!std::isfinite(pa) || pa <= tol) return false;
+        if (piv != k) {
+            for (size_t j = k; j < n; ++j) std::swap(m[k * n + j], m[piv * n + j]);
+            std::swap(b[k], b[piv]);
+        }
+        const double dg = m[k * n + k];
+        for (size_t i = k + 1; i < n; ++i) {
+            const double f = m[i * n + k] / dg;
+            if (!std::isfinite(f)) return false;
+            m[i * n + k] = 0.0;
+            for (size_t j = k + 1; j < n; ++j) m[i * n + j] -= f * m[k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    std::vector<double> s(n, 0.0);
+    for (size_t ii = n; ii-- > 0;) {
+        double sum = b[ii];
+        for (size_t j = ii + 1; j < n; ++j) sum -= m[ii * n + j] * s[j];
+        const double dg = m[ii * n + ii];
+        if (!std::isfinite(dg) || std::abs(dg) <= tol) return false;
+        s[ii] = sum / dg;
+        if (!std::isfinite(s[ii])) return false;
+    }
+    for (size_t i = 0; i < n; ++i) x[i] = s[i];
+    return true;
+}
+
+// vcycle: core/linalg/amg_preconditioner.h:2411-2443.
+bool vcycle(const Amg& h, size_t l, const std::vector<double>& r, std::vector<double>& x) {
+    if (l + 1 == h.lv.size()) return smooth_coarsest(h, l, r, x);
+    const Level& L = h.lv[l];
+    if (!smooth(h, l, r, x, h.pre)) return false;
+    std::vector<double> Ax(r.size());
+    matvec(L.A, x.data(), Ax.data());
+    const size_t nc = h.lv[l + 1].A.n;
+    std::vector<double> cr(nc, 0.0), cx(nc, 0.0);
+    for (size_t i = 0; i < r.size(); ++i) {
+        const double res = r[i] - Ax[i];
+        for (const auto& e : L.P[i]) cr[e.first] += e.second * res;
+    }
+    if (!vcycle(h, l + 1, cr, cx)) return false;
+    for (size_t i = 0; i < r.size(); ++i) {
+        for (const auto& e : L.P[i]) x[i] += e.second * cx[e.first];
+        if (!std::isfinite(x[i])) return false;
+    }
+    return smooth(h, l, r, x, h.post);
+}
+
+// apply: core/linalg/amg_preconditioner.h:1191-1203 (no null space).
+bool amg_apply(const Amg& h, const double* r, double* z) {
+    if (h.lv.empty()) return false;
+    const size_t n = h.lv[0].A.n;
+    std::vector<double> rr(r, r + n), x(n, 0.0);
+    for (double v : rr)
+        if (!std::isfinite(v)) return false;
+    if (!vcycle(h, 0, rr, x)) return false;
+    for (size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(x[i])) return false;
+        z[i] = x[i];
+    }
+    return true;
+}
+
+}  // namespace
+
+// ============================================================ C ABI =======
+
+extern "C" {
+
+struct oracle_result {
+    uint32_t status;  // SolverStatus: 0 converged, 1 max-iter, 2 diverged, 3 not applicable
+    uint32_t pad;
+    uint64_t iterations;
+    double residual;
+    double residual_relative;
+    uint64_t residual_replacements;
+};
+
+int oracle_geometry(size_t np, const double* px, const double* py, const double* pz,
+                    size_t nf, const uint32_t* foff, const uint32_t* fvtx,
+                    const uint32_t* owner, const int32_t* neigh, size_t nc,
+                    const uint32_t* coff, const uint32_t* cfaces, double* face_centre,
+                    double* face_sf, double* cell_centre, double* cell_volume) {
+    Geometry g;
+    if (!build_geometry(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces, g))
+        return 1;
+    for (size_t f = 0; f < nf; ++f) {
+        face_centre[3 * f] = g.face_centre[f].x;
+        face_centre[3 * f + 1] = g.face_centre[f].y;
+        face_centre[3 * f + 2] = g.face_centre[f].z;
+        face_sf[3 * f] = g.face_sf[f].x;
+        face_sf[3 * f + 1] = g.face_sf[f].y;
+        face_sf[3 * f + 2] = g.face_sf[f].z;
+    }
+    for (size_t c = 0; c < nc; ++c) {
+        cell_centre[3 * c] = g.cell_centre[c].x;
+        cell_centre[3 * c + 1] = g.cell_centre[c].y;
+        cell_centre[3 * c + 2] = g.cell_centre[c].z;
+        cell_volume[c] = g.cell_volume[c];
+    }
+    return 0;
+}
+
+// Geometry arrays below are AoS xyz triplets as produced by oracle_geometry.
+static V3 at(const double* a, size_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+
+// gauss_gradient_with_boundary (default one-sweep, distance weights,
+// non-projected): physics/steady_incompressible_solver.h:1634-1750.
+// patch_of_face: patch id per face (>= n_patches for internal faces);
+// patch_kind: 0 = ordinary, 1 = EMPTY (skipped), 2 = SYMMETRY;
+// bc_type per patch: 0 = none/zero-gradient, 1 = fixed value, 2 = fixed gradient.
+int oracle_gauss_gradient(size_t nf, const uint32_t* owner, const int32_t* neigh, size_t nc,
+                          const uint32_t* coff, const uint32_t* cfaces, const double* fc,
+                          const double* fs, const double* cc, const double* vol,
+                          const uint32_t* patch_of_face, size_t n_patches,
+                          const uint8_t* patch_kind, const uint8_t* bc_type,
+                          const double* bc_value, const double* p, double* grad /*3 x nc SoA*/) {
+    (void)nf;
+    for (size_t c = 0; c < nc; ++c) {
+        V3 sum{};
+        for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+            const uint32_t f = cfaces[k];
+            const bool own = owner[f] == c;
+            const V3 Sf = own ? at(fs, f) : scale(at(fs, f), -1.0);
+            double vf = p[c];
+            if (neigh[f] >= 0) {
+                const size_t other = own ? size_t(neigh[f]) : size_t(owner[f]);
+                const double dc = mag(sub(at(fc, f), at(cc, c)));
+                const double dn = mag(sub(at(cc, other), at(fc, f)));
+                if (!(dc + dn > 0.0)) return 1;
+                const double w = dn / (dc + dn);
+                vf = w * p[c] + (1.0 - w) * p[other];
+            } else {
+                const uint32_t pp = patch_of_face[f];
+                if (pp < n_patches && patch_kind[pp] == 1) continue;
+                if (pp < n_patches) {
+                    if (bc_type[pp] == 1) {
+                        vf = bc_value[pp];
+                    } else if (bc_type[pp] == 2) {
+                        const V3 out = scale(Sf, 1.0 / mag(Sf));
+                        const V3 cf = sub(at(fc, f), at(cc, c));
+                        vf += bc_value[pp] * dot(cf, out);
+                    }
+                }
+            }
+            sum = add(sum, scale(Sf, vf));
+        }
+        const double invV = 1.0 / vol[c];
+        grad[c] = sum.x * invV;
+        grad[nc + c] = sum.y * invV;
+        grad[2 * nc + c] = sum.z * invV;
+    }
+    return 0;
+}
+
+// make_mass_flux (non-projected weights): physics/steady_incompressible_solver.h:1842-1905.
+// vbc_type per patch: 0 = zero-gradient, 1 = fixed value with vbc_value[3*patch].
+int oracle_mass_flux(size_t nf, const uint32_t* owner, const int32_t* neigh, size_t nc,
+                     const double* fs, const uint32_t* patch_of_face, size_t n_patches,
+                     const uint8_t* patch_kind, const uint8_t* vbc_type, const double* vbc_value,
+                     double rho, const double* U /*3 x nc SoA*/, double* phi) {
+    for (size_t f = 0; f < nf; ++f) {
+        const uint32_t o = owner[f];
+        V3 Uf{U[o], U[nc + o], U[2 * nc + o]};
+        if (neigh[f] >= 0) {
+            const size_t n = size_t(neigh[f]);
+            const V3 Un{U[n], U[nc + n], U[2 * nc + n]};
+            const double wn = 0.5;
+            Uf = add(scale(Uf, 1.0 - wn), scale(Un, wn));
+        } else {
+            const uint32_t pp = patch_of_face[f];
+            if (pp < n_patches && (patch_kind[pp] == 1 || patch_kind[pp] == 2)) {
+                phi[f] = 0.0;
+                continue;
+            }
+            if (pp < n_patches && vbc_type[pp] == 1)
+                Uf = {vbc_value[3 * pp], vbc_value[3 * pp + 1], vbc_value[3 * pp + 2]};
+        }
+        phi[f] = rho * dot(Uf, at(fs, f));
+    }
+    return 0;
+}
+
+// pressure_face_orthogonal_coefficient: steady_incompressible_solver.h:1914-1935.
+static bool ortho_coeff(const V3& Sf, const V3& dv, const V3& dAU, double& out) {
+    const double d = mag(dv);
+    const double area = mag(Sf);
+    if (!(d > 0.0) || !std::isfinite(d) || !(area > 0.0) || !std::isfinite(area)) return false;
+    const V3 e{dv.x / d, dv.y / d, dv.z / d};
+    const double oa = dot(Sf, e);
+    if (!(oa > 0.0) || !std::isfinite(oa)) return false;
+    const double rfn = dAU.x * e.x * e.x + dAU.y * e.y * e.y + dAU.z * e.z * e.z;
+    out = rfn * oa / d;
+    return std::isfinite(out);
+}
+
+// pressure_face_nonorthogonal_flux: steady_incompressible_solver.h:1941-1965.
+static bool nonortho_flux(const V3& Sf, const V3& dv, const V3& dAU, const V3& g, double& out) {
+    const double d = mag(dv);
+    if (!(d > 0.0) || !std::isfinite(d)) return false;
+    const V3 e{dv.x / d, dv.y / d, dv.z / d};
+    const double oa = dot(Sf, e);
+    const double rfn = dAU.x * e.x * e.x + dAU.y * e.y * e.y + dAU.z * e.z * e.z;
+    const double full = dAU.x * g.x * Sf.x + dAU.y * g.y * Sf.y + dAU.z * g.z * Sf.z;
+    const double impl = rfn * dot(g, e) * oa;
+    out = full - impl;
+    return std::isfinite(out);
+}
+
+// SIMPLE pressure-corrector assembly with default controls (SIMPLE, w = 1/2,
+// no compensated assembly): steady_incompressible_solver.h:8048-8181
+// (continuity cell loop :8080-8100 calling rhie_chow_pressure_flux_internal
+// :1967-2029; coefficient face loop :8102-8122 with the lambda :6485-6499;
+// diagonal/RHS :8124-8129; fixed-pressure faces :8131-8181), then the CSR
+// build and reference-cell elimination :8599-8652.
+// rAU: 3 x nc SoA. grad_p: 3 x nc SoA (from oracle_gauss_gradient).
+// reference_cell >= nc disables elimination (fixed-pressure systems).
+// Outputs: CSR (rows = nc or nc-1) and rhs; *n_out, *nnz_out written.
+// If rowp is null the call only reports sizes.
+int oracle_pressure_assemble(size_t nf, const uint32_t* owner, const int32_t* neigh, size_t nc,
+                             const uint32_t* coff, const uint32_t* cfaces, const double* fc,
+                             const double* fs, const double* cc, const double* vol,
+                             const uint32_t* patch_of_face, size_t n_patches,
+                             const uint8_t* pbc_type, const double* pbc_value, double rho,
+                             const double* rAU, const double* phiHbyA, const double* p,
+                             const double* grad_p, size_t reference_cell, size_t* n_out,
+                             size_t* nnz_out, uint32_t* rowp, uint32_t* cols, double* vals,
+                             double* rhs) {
+    auto dAU_face = [&](size_t o, size_t n, V3& out) {
+        const double wn = 0.5, wo = 1.0 - wn;
+        out.x = wo * vol[o] * rAU[o] + wn * vol[n] * rAU[n];
+        out.y = wo * vol[o] * rAU[nc + o] + wn * vol[n] * rAU[nc + n];
+        out.z = wo * vol[o] * rAU[2 * nc + o] + wn * vol[n] * rAU[2 * nc + n];
+    };
+    auto gface = [&](size_t o, size_t n) {
+        const double wn = 0.5, wo = 1.0 - wn;
+        return V3{wo * grad_p[o] + wn * grad_p[n], wo * grad_p[nc + o] + wn * grad_p[nc + n],
+                  wo * grad_p[2 * nc + o] + wn * grad_p[2 * nc + n]};
+    };
+    // rhie_chow_pressure_flux_internal :1967-2029.
+    auto rc_internal = [&](size_t f, double& out) {
+        const size_t o = owner[f], n = size_t(neigh[f]);
+        const V3 Sf = at(fs, f);
+        const V3 dv = sub(at(cc, n), at(cc, o));
+        V3 dAU;
+        dAU_face(o, n, dAU);
+        const V3 gp = gface(o, n);
+        double ic = 0.0, nof = 0.0;
+        if (!ortho_coeff(Sf, dv, dAU, ic) || !nonortho_flux(Sf, dv, dAU, gp, nof)) return false;
+        out = rho * (ic * (p[n] - p[o]) + nof);
+        return true;
+    };
+    std::vector<std::map<size_t, double>> rows(nc);
+    std::vector<double> diag(nc, 0.0), cont(nc, 0.0), b(nc, 0.0);
+    for (size_t c = 0; c < nc; ++c) {
+        for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+            const uint32_t f = cfaces[k];
+            cont[c] += owner[f] == c ? phiHbyA[f] : -phiHbyA[f];
+            if (neigh[f] >= 0) {
+                double pn = 0.0;
+                if (!rc_internal(f, pn)) return 1;
+                cont[c] += owner[f] == c ? -pn : pn;
+            }
+        }
+    }
+    for (size_t f = 0; f < nf; ++f) {
+        if (neigh[f] < 0) continue;
+        const size_t o = owner[f], n = size_t(neigh[f]);
+        const V3 Sf = at(fs, f);
+        const V3 dv = sub(at(cc, n), at(cc, o));
+        V3 dAU;
+        dAU_face(o, n, dAU);
+        double raw = 0.0;
+        if (!ortho_coeff(Sf, dv, dAU, raw)) return 1;
+        const double coeff = rho * raw;
+        diag[o] += coeff;
+        diag[n] += coeff;
+        rows[o][n] -= coeff;
+        rows[n][o] -= coeff;
+    }
+    for (size_t c = 0; c < nc; ++c) {
+        rows[c][c] += diag[c];
+        b[c] = -cont[c];
+    }
+    bool fixed_p = false;
+    for (size_t f = 0; f < nf; ++f) {
+        if (neigh[f] >= 0) continue;
+        const uint32_t pp = patch_of_face[f];
+        if (pp >= n_patches || pbc_type[pp] != 1) continue;
+        fixed_p = true;
+        const size_t o = owner[f];
+        const V3 dv = sub(at(fc, f), at(cc, o));
+        const V3 Sf = at(fs, f);
+        const V3 dAU{vol[o] * rAU[o], vol[o] * rAU[nc + o], vol[o] * rAU[2 * nc + o]};
+        double bc = 0.0, def = 0.0;
+        if (!ortho_coeff(Sf, dv, dAU, bc)) return 1;
+        rows[o][o] += rho * bc;
+        const V3 gp{grad_p[o], grad_p[nc + o], grad_p[2 * nc + o]};
+        if (!nonortho_flux(Sf, dv, dAU, gp, def)) return 1;
+        b[o] += rho * (bc * (pbc_value[pp] - p[o]) + def);
+    }
+    const bool eliminate = !fixed_p && nc > 1 && reference_cell < nc;
+    Coo coo;
+    std::vector<double> bb;
+    if (!eliminate) {
+        coo.n = nc;
+        for (size_t r = 0; r < nc; ++r)
+            for (const auto& kv : rows[r]) coo.push(r, kv.first, kv.second);
+        bb = b;
+    } else {
+        const size_t ref = reference_cell;
+        auto ri = [ref](size_t c) { return c < ref ? c : c - 1; };
+        coo.n = nc - 1;
+        bb.assign(nc - 1, 0.0);
+        for (size_t r = 0; r < nc; ++r) {
+            if (r == ref) continue;
+            bb[ri(r)] = b[r];
+            for (const auto& kv : rows[r]) {
+                if (kv.first == ref) continue;
+                coo.push(ri(r), ri(kv.first), kv.second);
+            }
+        }
+    }
+    Csr A;
+    if (!coo.finalize(A)) return 1;
+    *n_out = A.n;
+    *nnz_out = A.nnz();
+    if (!rowp) return 0;
+    std::copy(A.rowp.begin(), A.rowp.end(), rowp);
+    std::copy(A.cols.begin(), A.cols.end(), cols);
+    std::copy(A.vals.begin(), A.vals.end(), vals);
+    std::copy(bb.begin(), bb.end(), rhs);
+    return 0;
+}
+
+// assemble_cell_diffusion_matrix: core/solvers/scalar_diffusion.h:39-163.
+// face_value: per-face prescribed value (NaN = unprescribed);
+// face_is_neumann: 1 if the prescribed value is an outward flux q_n.
+// Output pattern is written when rowp is non-null; sizes always.
+int oracle_diffusion_assemble(size_t nf, const uint32_t* owner, const int32_t* neigh, size_t nc,
+                              const double* fc, const double* fs, const double* cc,
+                              const double* vol, double gamma, const double* source,
+                              const double* face_value, const uint8_t* face_is_neumann,
+                              size_t* nnz_out, uint32_t* rowp, uint32_t* cols, double* vals,
+                              double* rhs) {
+    std::vector<double> b(nc);
+    for (size_t c = 0; c < nc; ++c) b[c] = source[c] * vol[c];
+    std::vector<std::vector<std::pair<size_t, double>>> rows(nc);
+    for (size_t c = 0; c < nc; ++c) rows[c].push_back({c, 0.0});
+    for (size_t f = 0; f < nf; ++f) {
+        const size_t o = owner[f];
+        const double area = mag(at(fs, f));
+        if (!(area > 0.0)) return 1;
+        if (neigh[f] >= 0) {
+            const size_t j = size_t(neigh[f]);
+            const double d = mag(sub(at(cc, j), at(cc, o)));
+            if (!(d > std::numeric_limits<double>::epsilon())) return 1;
+            const double g = gamma * area / d;
+            rows[o][0].second += g;
+            rows[j][0].second += g;
+            rows[o].push_back({j, -g});
+            rows[j].push_back({o, -g});
+        } else if (std::isfinite(face_value[f])) {
+            const double d = mag(sub(at(fc, f), at(cc, o)));
+            if (!(d > std::numeric_limits<double>::epsilon())) return 1;
+            const double g = gamma * area / d;
+            if (!face_is_neumann[f]) {
+                rows[o][0].second += g;
+                b[o] += g * face_value[f];
+            } else {
+                b[o] += face_value[f] * area;
+            }
+        }
+    }
+    Coo coo;
+    coo.n = nc;
+    for (size_t i = 0; i < nc; ++i) {
+        std::stable_sort(rows[i].begin(), rows[i].end(),
+                         [](const auto& a, const auto& b2) { return a.first < b2.first; });
+        for (size_t k = 0; k < rows[i].size();) {
+            const size_t col = rows[i][k].first;
+            double v = 0.0;
+            while (k < rows[i].size() && rows[i][k].first == col) v += rows[i][k++].second;
+            if (v != 0.0 || col == i) coo.push(i, col, v);
+        }
+    }
+    Csr A;
+    if (!coo.finalize(A)) return 1;
+    *nnz_out = A.nnz();
+    if (!rowp) return 0;
+    std::copy(A.rowp.begin(), A.rowp.end(), rowp);
+    std::copy(A.cols.begin(), A.cols.end(), cols);
+    std::copy(A.vals.begin(), A.vals.end(), vals);
+    std::copy(b.begin(), b.end(), rhs);
+    return 0;
+}
+
+void oracle_spmv(size_t n, const uint32_t* rowp, const uint32_t* cols, const double* vals,
+                 const double* x, double* y) {
+    for (size_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (uint32_t k = rowp[i]; k < rowp[i + 1]; ++k) s += vals[k] * x[cols[k]];
+        y[i] = s;
+    }
+}
+
+double oracle_norm2(const double* a, size_t n) { return knorm2(a, n); }
+
+// ---- AMG handle ----
+void* oracle_amg_create(int method, int policy, int require_certificate) {
+    Amg* h = new Amg();
+    h->method = method;
+    // NativeAMGPreconditioner::setup policies: hypre_amg.cpp:172-186, :208.
+    if (policy == 0) {
+        h->cfg_omega = 0.65;
+        h->pre = h->post = 2;
+        h->max_levels = 15;
+    } else if (policy == 2) {
+        h->cfg_omega = 0.75;
+    }
+    h->omega = h->cfg_omega;
+    h->require_cert = require_certificate != 0;
+    return h;
+}
+void oracle_amg_destroy(void* h) { delete static_cast<Amg*>(h); }
+// MatrixFreeVcyclePreconditioner constructor knobs (amg_preconditioner.h:42-76):
+// strength threshold and prolongator filter.
+void oracle_amg_configure(void* h, double theta, double filter) {
+    static_cast<Amg*>(h)->theta = theta;
+    static_cast<Amg*>(h)->filter = filter;
+}
+
+int oracle_amg_setup(void* hp, size_t n, const uint32_t* rowp, const uint32_t* cols,
+                     const double* vals) {
+    Amg* h = static_cast<Amg*>(hp);
+    Csr A;
+    A.n = n;
+    A.rowp.assign(rowp, rowp + n + 1);
+    A.cols.assign(cols, cols + rowp[n]);
+    A.vals.assign(vals, vals + rowp[n]);
+    return amg_setup(*h, A) ? 0 : 1;
+}
+int oracle_amg_update_values(void* hp, const double* vals) {
+    return amg_update(*static_cast<Amg*>(hp), vals) ? 0 : 1;
+}
+size_t oracle_amg_num_levels(void* hp) { return static_cast<Amg*>(hp)->lv.size(); }
+double oracle_amg_omega(void* hp) { return static_cast<Amg*>(hp)->omega; }
+int oracle_amg_level_size(void* hp, size_t l, size_t* n, size_t* nnz, size_t* p_nnz) {
+    Amg* h = static_cast<Amg*>(hp);
+    if (l >= h->lv.size()) return 1;
+    *n = h->lv[l].A.n;
+    *nnz = h->lv[l].A.nnz();
+    size_t pn = 0;
+    for (const auto& r : h->lv[l].P) pn += r.size();
+    *p_nnz = pn;
+    return 0;
+}
+int oracle_amg_level_matrix(void* hp, size_t l, uint32_t* rowp, uint32_t* cols, double* vals) {
+    Amg* h = static_cast<Amg*>(hp);
+    if (l >= h->lv.size()) return 1;
+    const Csr& A = h->lv[l].A;
+    std::copy(A.rowp.begin(), A.rowp.end(), rowp);
+    std::copy(A.cols.begin(), A.cols.end(), cols);
+    std::copy(A.vals.begin(), A.vals.end(), vals);
+    return 0;
+}
+int oracle_amg_level_transfer(void* hp, size_t l, uint64_t* aggregate, uint32_t* prowp,
+                              uint32_t* pcols, double* pvals) {
+    Amg* h = static_cast<Amg*>(hp);
+    if (l + 1 >= h->lv.size()) return 1;
+    const Level& L = h->lv[l];
+    for (size_t i = 0; i < L.A.n; ++i) aggregate[i] = L.aggregate[i];
+    size_t k = 0;
+    prowp[0] = 0;
+    for (size_t i = 0; i < L.A.n; ++i) {
+        for (const auto& e : L.P[i]) {
+            pcols[k] = uint32_t(e.first);
+            pvals[k] = e.second;
+            ++k;
+        }
+        prowp[i + 1] = uint32_t(k);
+    }
+    return 0;
+}
+int oracle_amg_apply(void* hp, const double* r, double* z) {
+    return amg_apply(*static_cast<Amg*>(hp), r, z) ? 0 : 1;
+}
+
+// solve_cg_impl with an external prepared preconditioner, adaptive residual
+// replacement, FP64 everywhere, no null space: core/linalg/cg_solver.h:45-569
+// (entry point solve_cg_prepared :653-667).
+int oracle_pcg(void* hp, const double* b, double* x, size_t max_iter, double tol,
+               oracle_result* res) {
+    Amg* h = static_cast<Amg*>(hp);
+    const Csr& A = h->lv.at(0).A;
+    const size_t n = A.n;
+    std::vector<double> r(n), rv(n), z(n), p(n), Ap(n), ax(n);
+    *res = oracle_result{};
+    auto true_res = [&]() {
+        matvec(A, x, ax.data());
+        for (size_t i = 0; i < n; ++i) rv[i] = b[i] - ax[i];
+        return knorm2(rv.data(), n);
+    };
+    const double bn = knorm2(b, n);
+    const double r0 = true_res();
+    r = rv;
+    const double ref = bn > 0.0 ? bn : r0;
+    const double tol_abs = tol * ref;
+    auto finish = [&](uint32_t st, size_t it, double rr) {
+        res->status = st;
+        res->iterations = it;
+        res->residual = rr;
+        res->residual_relative = ref > 0.0 ? rr / ref : 0.0;
+        return 0;
+    };
+    if (r0 <= tol_abs) return finish(0, 0, r0);
+    auto precond = [&]() {
+        if (!amg_apply(*h, r.data(), z.data())) return false;
+        for (double v : z)
+            if (!std::isfinite(v)) return false;
+        return true;
+    };
+    if (!precond()) return finish(3, 0, true_res());
+    p = z;
+    double rsold = kdot(r.data(), z.data(), n);
+    if (!std::isfinite(rsold)) return finish(2, 0, true_res());
+    if (!(rsold > 0.0)) return finish(3, 0, true_res());
+    double last_repl = std::numeric_limits<double>::infinity();
+    for (size_t it = 1; it <= max_iter; ++it) {
+        matvec(A, p.data(), Ap.data());
+        const double pAp = kdot(p.data(), Ap.data(), n);
+        if (!(pAp > 0.0) || !std::isfinite(pAp)) return finish(3, it, true_res());
+        const double alpha = rsold / pAp;
+        if (!std::isfinite(alpha)) return finish(2, it, true_res());
+        for (size_t i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+        }
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(x[i]) || !std::isfinite(r[i])) return finish(2, it, true_res());
+        if (!precond()) return finish(3, it, true_res());
+        const double rsnew = kdot(r.data(), z.data(), n);
+        if (!std::isfinite(rsnew)) return finish(2, it, true_res());
+        const double tr = true_res();
+        if (tr <= tol_abs) return finish(0, it, tr);
+        double rec2 = 0.0;
+        for (size_t i = 0; i < n; ++i) rec2 += r[i] * r[i];
+        const double rec = std::sqrt(rec2);
+        const double gap = std::abs(tr - rec);
+        const double sc = std::max(tr, rec);
+        const bool progress = tr < 0.5 * last_repl;
+        const bool collapsed = !(rsnew > 0.0);
+        const bool adaptive = std::isfinite(rec) && sc > 0.0 && gap > 0.25 * sc && progress;
+        const bool repair = collapsed && progress;
+        if (collapsed && !repair) return finish(1, it, tr);
+        if (adaptive || repair) {
+            ++res->residual_replacements;
+            last_repl = tr;
+            r = rv;
+            if (!precond()) return finish(3, it, true_res());
+            const double rz = kdot(r.data(), z.data(), n);
+            if (!std::isfinite(rz) || !(rz > 0.0)) return finish(3, it, true_res());
+            if (tr <= tol_abs) return finish(0, it, tr);
+            p = z;
+            rsold = rz;
+            continue;
+        }
+        const double beta = rsnew / rsold;
+        for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        rsold = rsnew;
+    }
+    return finish(1, max_iter, true_res());
+}
+
+// Restarted right-preconditioned FGMRES with two MGS passes, Givens
+// rotations, best-iterate rollback and adaptive restart:
+// core/linalg/gmres_solver.h:28-600 with KrylovControls defaults
+// (core/linalg/krylov_controls.h:10-53). The periodic candidate check
+// (:367-431) is included.
+int oracle_fgmres(void* hp, const double* b, double* x, int restart, size_t max_iter, double tol,
+                  oracle_result* res) {
+    Amg* h = static_cast<Amg*>(hp);
+    const Csr& A = h->lv.at(0).A;
+    const size_t n = A.n;
+    *res = oracle_result{};
+    const int rmin = 10, rmax = 512;
+    int cur = std::clamp(restart, rmin, rmax);
+    if (n < size_t(cur)) cur = int(n);
+    const double bn = knorm2(b, n);
+    if (!std::isfinite(bn) || bn == 0.0) {
+        res->status = 3;
+        return 0;
+    }
+    const double rhs_scale = std::max(bn, 1e-300);
+    const double tolv = tol * bn;
+    std::vector<double> ax(n), r(n), w(n);
+    auto true_res = [&]() {
+        matvec(A, x, ax.data());
+        for (size_t i = 0; i < n; ++i) r[i] = b[i] - ax[i];
+        return knorm2(r.data(), n);
+    };
+    double beta = true_res();
+    auto finish = [&](uint32_t st, size_t it, double rr) {
+        res->status = st;
+        res->iterations = it;
+        res->residual = rr;
+        res->residual_relative = rr / rhs_scale;
+        return 0;
+    };
+    if (!std::isfinite(beta)) return finish(2, 0, beta);
+    if (beta <= tolv) return finish(0, 0, beta);
+    size_t its = 0;
+    double prev_cycle = beta;
+    std::vector<double> best_x(x, x + n);
+    double best = beta;
+    while (its < max_iter) {
+        const size_t m = size_t(cur);
+        std::vector<std::vector<double>> V(m + 1, std::vector<double>(n)), Z(m, std::vector<double>(n));
+        std::vector<double> H((m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0), y(m, 0.0);
+        auto Hr = [&](size_t i, size_t j) -> double& { return H[i * m + j]; };
+        beta = true_res();
+        if (!std::isfinite(beta)) {
+            std::copy(best_x.begin(), best_x.end(), x);
+            return finish(2, its, best);
+        }
+        if (beta <= tolv) break;
+        for (size_t i = 0; i < n; ++i) V[0][i] = r[i] / beta;
+        g[0] = beta;
+        size_t used = 0;
+        double est = beta;
+        bool brk = false, reliable_restart = false;
+        for (size_t j = 0; j < m && its < max_iter; ++j) {
+            if (!amg_apply(*h, V[j].data(), Z[j].data())) {
+                std::copy(best_x.begin(), best_x.end(), x);
+                return finish(3, its, best);
+            }
+            matvec(A, Z[j].data(), w.data());
+            const double an = knorm2(w.data(), n);
+            if (!std::isfinite(an)) {
+                std::copy(best_x.begin(), best_x.end(), x);
+                return finish(2, its, best);
+            }
+            for (size_t i = 0; i <= j; ++i) {
+                const double hh = kdot(V[i].data(), w.data(), n);
+                Hr(i, j) = hh;
+                for (size_t k = 0; k < n; ++k) w[k] -= hh * V[i][k];
+            }
+            for (size_t i = 0; i <= j; ++i) {
+                const double cc = kdot(V[i].data(), w.data(), n);
+                Hr(i, j) += cc;
+                for (size_t k = 0; k < n; ++k) w[k] -= cc * V[i][k];
+            }
+            const double hn = knorm2(w.data(), n);
+            if (!std::isfinite(hn)) {
+                std::copy(best_x.begin(), best_x.end(), x);
+                return finish(2, its, best);
+            }
+            Hr(j + 1, j) = hn;
+            const double floor_ = 4096.0 * std::numeric_limits<double>::epsilon() * std::max(an, 1e-300);
+            const bool bd = hn <= floor_ && an > 0.0;
+            if (hn > 0.0)
+                for (size_t k = 0; k < n; ++k) V[j + 1][k] = w[k] / hn;
+            for (size_t i = 0; i < j; ++i) {
+                const double h0 = Hr(i, j), h1 = Hr(i + 1, j);
+                Hr(i, j) = cs[i] * h0 + sn[i] * h1;
+                Hr(i + 1, j) = -sn[i] * h0 + cs[i] * h1;
+            }
+            const double a = Hr(j, j), b2 = Hr(j + 1, j);
+            const double rho = std::hypot(a, b2);
+            if (rho > 0.0) {
+                cs[j] = a / rho;
+                sn[j] = b2 / rho;
+                Hr(j, j) = rho;
+                Hr(j + 1, j) = 0.0;
+                const double gj = g[j], gj1 = g[j + 1];
+                g[j] = cs[j] * gj + sn[j] * gj1;
+                g[j + 1] = -sn[j] * gj + cs[j] * gj1;
+            }
+            ++its;
+            ++used;
+            est = std::abs(g[j + 1]);
+            if (bd) {
+                brk = true;
+                break;
+            }
+            if (est <= tolv) break;
+            if (its % 50 == 0) {
+                std::vector<double> co(used, 0.0);
+                bool usable = true;
+                for (size_t i = used; i-- > 0;) {
+                    double s = g[i];
+                    for (size_t k = i + 1; k < used; ++k) s -= Hr(i, k) * co[k];
+                    const double dg = Hr(i, i);
+                    if (!std::isfinite(s) || !std::isfinite(dg) ||
+                        std::abs(dg) <= 128.0 * std::numeric_limits<double>::epsilon() * std::max(an, 1e-300)) {
+                        usable = false;
+                        break;
+                    }
+                    co[i] = s / dg;
+                }
+                if (usable) {
+                    std::vector<double> trial(x, x + n), ax2(n), rt(n);
+                    for (size_t k = 0; k < used; ++k)
+                        for (size_t i = 0; i < n; ++i) trial[i] += co[k] * Z[k][i];
+                    matvec(A, trial.data(), ax2.data());
+                    for (size_t i = 0; i < n; ++i) rt[i] = b[i] - ax2[i];
+                    const double ver = knorm2(rt.data(), n);
+                    if (std::isfinite(ver)) {
+                        if (ver / std::max(est, 1e-300) > 10.0) {
+                            reliable_restart = true;
+                            break;
+                        }
+                    } else {
+                        reliable_restart = true;
+                        break;
+                    }
+                }
+            }
+        }
+        if (used == 0) break;
+        std::fill(y.begin(), y.end(), 0.0);
+        for (size_t i = used; i-- > 0;) {
+            double s = g[i];
+            for (size_t j = i + 1; j < used; ++j) s -= Hr(i, j) * y[j];
+            const double dg = Hr(i, i);
+            double hs = std::abs(dg);
+            for (size_t k = 0; k < used; ++k) hs = std::max(hs, std::abs(Hr(k, i)));
+            const double fl = 128.0 * std::numeric_limits<double>::epsilon() * std::max(hs, 1e-300);
+            if (std::abs(dg) <= fl) {
+                if (std::abs(s) <= fl) {
+                    y[i] = 0.0;
+                    continue;
+                }
+                double br = true_res();
+                if (!std::isfinite(br) || best < br) {
+                    std::copy(best_x.begin(), best_x.end(), x);
+                    br = best;
+                }
+                return finish(2, its, br);
+            }
+            y[i] = s / dg;
+        }
+        for (size_t j = 0; j < used; ++j)
+            for (size_t i = 0; i < n; ++i) x[i] += y[j] * Z[j][i];
+        beta = true_res();
+        if (reliable_restart && std::isfinite(beta)) ++res->residual_replacements;
+        if (!std::isfinite(beta)) {
+            std::copy(best_x.begin(), best_x.end(), x);
+            return finish(2, its, best);
+        }
+        if (beta < best) {
+            best = beta;
+            std::copy(x, x + n, best_x.begin());
+        }
+        if (beta <= tolv) return finish(0, its, beta);
+        if (brk) {
+            if (best < beta) {
+                std::copy(best_x.begin(), best_x.end(), x);
+                beta = best;
+            }
+            return finish(2, its, beta);
+        }
+        const double red = beta / std::max(prev_cycle, 1e-300);
+        if (!std::isfinite(red)) {
+            std::copy(best_x.begin(), best_x.end(), x);
+            return finish(2, its, best);
+        }
+        // choose_gmres_restart: core/linalg/krylov_controls.h:28-51.
+        if (red > 0.1 * 2.0)
+            cur = cur > rmax - 5 ? rmax : cur + 5;
+        else if (red < 0.1 * 0.5)
+            cur = std::max(rmin, cur - 5);
+        else
+            cur = std::clamp(cur, rmin, rmax);
+        if (n < size_t(cur)) cur = int(n);
+        prev_cycle = beta;
+    }
+    if (best < beta) {
+        std::copy(best_x.begin(), best_x.end(), x);
+        beta = best;
+    }
+    return finish(1, its, beta);
+}
+
+}  // extern "C"

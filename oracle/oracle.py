@@ -1,0 +1,272 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+``Oracle()`` loads oracle/liboracle.so, the FP64 restatement in
+cfdx_oracle.cpp. ``Oracle(reference=True)`` loads oracle/_ref/libcfdxref.so,
+the reference's own routines compiled from its sources, when that build
+exists. Both expose the same calls, so a test can run one case through the
+restatement, the reference and the HIP product and compare.
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py import this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcfdxref.so")
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+sz = C.c_size_t
+vp = C.c_void_p
+
+
+class OracleResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("pad", C.c_uint32), ("iterations", C.c_uint64),
+                ("residual", C.c_double), ("residual_relative", C.c_double),
+                ("residual_replacements", C.c_uint64)]
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def _f(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Oracle:
+    def __init__(self, reference: bool = False):
+        path = REF_SO if reference else ORACLE_SO
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (build with make -C oracle)")
+        self.lib = C.CDLL(path)
+        self.p = "ref_" if reference else "oracle_"
+        self.reference = reference
+        L, p = self.lib, self.p
+        mesh_args = [sz, f64p, f64p, f64p, sz, u32p, u32p, u32p, i32p, sz, u32p, u32p]
+        getattr(L, p + "geometry").argtypes = mesh_args + [f64p, f64p, f64p, f64p]
+        getattr(L, p + "geometry").restype = C.c_int
+        for name in ("amg_create",):
+            getattr(L, p + name).restype = vp
+            getattr(L, p + name).argtypes = [C.c_int, C.c_int, C.c_int]
+        getattr(L, p + "amg_destroy").argtypes = [vp]
+        getattr(L, p + "amg_configure").argtypes = [vp, C.c_double, C.c_double]
+        getattr(L, p + "amg_setup").argtypes = [vp, sz, u32p, u32p, f64p]
+        getattr(L, p + "amg_update_values").argtypes = [vp, f64p]
+        getattr(L, p + "amg_num_levels").argtypes = [vp]
+        getattr(L, p + "amg_num_levels").restype = sz
+        getattr(L, p + "amg_omega").argtypes = [vp]
+        getattr(L, p + "amg_omega").restype = C.c_double
+        getattr(L, p + "amg_level_size").argtypes = [vp, sz, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]
+        getattr(L, p + "amg_level_matrix").argtypes = [vp, sz, u32p, u32p, f64p]
+        getattr(L, p + "amg_level_transfer").argtypes = [vp, sz, u64p, u32p, u32p, f64p]
+        getattr(L, p + "amg_apply").argtypes = [vp, f64p, f64p]
+        getattr(L, p + "pcg").argtypes = [vp, f64p, f64p, sz, C.c_double, C.POINTER(OracleResult)]
+        getattr(L, p + "fgmres").argtypes = [vp, f64p, f64p, C.c_int, sz, C.c_double, C.POINTER(OracleResult)]
+        if reference:
+            L.ref_geometry_cache.argtypes = mesh_args + [f64p, f64p, f64p, f64p]
+            L.ref_diffusion_assemble.argtypes = mesh_args + [C.c_double, f64p, f64p, u8p, C.POINTER(sz),
+                                                             vp, vp, vp, vp]
+            L.ref_pressure_assemble.argtypes = mesh_args + [
+                u32p, sz, u8p, u8p, f64p, u8p, f64p, C.c_double, f64p, f64p, f64p, sz, f64p, f64p,
+                C.POINTER(sz), C.POINTER(sz), vp, vp, vp, vp]
+        else:
+            L.oracle_gauss_gradient.argtypes = [sz, u32p, i32p, sz, u32p, u32p, f64p, f64p, f64p, f64p, u32p, sz,
+                                                u8p, u8p, f64p, f64p, f64p]
+            L.oracle_mass_flux.argtypes = [sz, u32p, i32p, sz, f64p, u32p, sz, u8p, u8p, f64p, C.c_double, f64p,
+                                           f64p]
+            L.oracle_pressure_assemble.argtypes = [sz, u32p, i32p, sz, u32p, u32p, f64p, f64p, f64p, f64p, u32p,
+                                                   sz, u8p, f64p, C.c_double, f64p, f64p, f64p, f64p, sz,
+                                                   C.POINTER(sz), C.POINTER(sz), vp, vp, vp, vp]
+            L.oracle_diffusion_assemble.argtypes = [sz, u32p, i32p, sz, f64p, f64p, f64p, f64p, C.c_double, f64p,
+                                                    f64p, u8p, C.POINTER(sz), vp, vp, vp, vp]
+            L.oracle_spmv.argtypes = [sz, u32p, u32p, f64p, f64p, f64p]
+            L.oracle_norm2.argtypes = [f64p, sz]
+            L.oracle_norm2.restype = C.c_double
+
+    # ---------------------------------------------------------- geometry --
+    @staticmethod
+    def _mesh_args(m):
+        return [len(m.px), m.px, m.py, m.pz, m.n_faces, m.face_offsets, m.face_vertices, m.owner, m.neighbour,
+                m.n_cells, m.cell_offsets, m.cell_faces]
+
+    def geometry(self, m, cache=False):
+        fc = np.empty(3 * m.n_faces)
+        fs = np.empty(3 * m.n_faces)
+        cc = np.empty(3 * m.n_cells)
+        vol = np.empty(m.n_cells)
+        fn = self.lib.ref_geometry_cache if cache else getattr(self.lib, self.p + "geometry")
+        if fn(*self._mesh_args(m), fc, fs, cc, vol) != 0:
+            raise RuntimeError("geometry failed")
+        return fc, fs, cc, vol
+
+    # -------------------------------------------------------- assembly --
+    def diffusion_assemble(self, m, geom, gamma, source, face_value, face_is_neumann):
+        fv = _f(face_value, np.float64)
+        fn_ = _f(face_is_neumann, np.uint8)
+        src = _f(source, np.float64)
+        nnz = sz()
+        n = m.n_cells
+        if self.reference:
+            args = self._mesh_args(m) + [gamma, src, fv, fn_]
+            call = self.lib.ref_diffusion_assemble
+        else:
+            fc, fs, cc, vol = geom
+            args = [m.n_faces, m.owner, m.neighbour, n, fc, fs, cc, vol, gamma, src, fv, fn_]
+            call = self.lib.oracle_diffusion_assemble
+        if call(*args, C.byref(nnz), None, None, None, None) != 0:
+            raise RuntimeError("diffusion assembly failed")
+        rp = np.empty(n + 1, np.uint32)
+        cols = np.empty(nnz.value, np.uint32)
+        vals = np.empty(nnz.value)
+        rhs = np.empty(n)
+        call(*args, C.byref(nnz), rp.ctypes.data_as(vp), cols.ctypes.data_as(vp), vals.ctypes.data_as(vp),
+             rhs.ctypes.data_as(vp))
+        return rp, cols, vals, rhs
+
+    def gauss_gradient(self, m, geom, bc_type, bc_value, p):
+        fc, fs, cc, vol = geom
+        g = np.empty(3 * m.n_cells)
+        st = self.lib.oracle_gauss_gradient(m.n_faces, m.owner, m.neighbour, m.n_cells, m.cell_offsets,
+                                            m.cell_faces, fc, fs, cc, vol, m.patch_of_face, m.n_patches,
+                                            m.patch_kind, _f(bc_type, np.uint8), _f(bc_value, np.float64),
+                                            _f(p, np.float64), g)
+        if st != 0:
+            raise RuntimeError("gradient failed")
+        return g
+
+    def mass_flux(self, m, geom, vbc_type, vbc_value, rho, U):
+        fc, fs, cc, vol = geom
+        phi = np.empty(m.n_faces)
+        self.lib.oracle_mass_flux(m.n_faces, m.owner, m.neighbour, m.n_cells, fs, m.patch_of_face, m.n_patches,
+                                  m.patch_kind, _f(vbc_type, np.uint8), _f(vbc_value, np.float64), rho,
+                                  _f(U, np.float64), phi)
+        return phi
+
+    def pressure_assemble(self, m, geom, pbc_type, pbc_value, rho, rAU, phi, p, grad_p, reference_cell,
+                          vbc_type=None, vbc_value=None, U=None):
+        """Oracle: takes phiHbyA and grad_p. Reference harness: takes U and
+        computes phi and grad_p itself; both are returned for comparison."""
+        n_out, nnz = sz(), sz()
+        nc = m.n_cells
+        if self.reference:
+            grad = np.empty(3 * nc)
+            phi_out = np.empty(m.n_faces)
+            args = self._mesh_args(m) + [m.patch_of_face, m.n_patches, m.patch_kind, _f(pbc_type, np.uint8),
+                                         _f(pbc_value, np.float64), _f(vbc_type, np.uint8),
+                                         _f(vbc_value, np.float64), rho, _f(rAU, np.float64), _f(U, np.float64),
+                                         _f(p, np.float64), reference_cell, grad, phi_out]
+            call = self.lib.ref_pressure_assemble
+        else:
+            fc, fs, cc, vol = geom
+            args = [m.n_faces, m.owner, m.neighbour, nc, m.cell_offsets, m.cell_faces, fc, fs, cc, vol,
+                    m.patch_of_face, m.n_patches, _f(pbc_type, np.uint8), _f(pbc_value, np.float64), rho,
+                    _f(rAU, np.float64), _f(phi, np.float64), _f(p, np.float64), _f(grad_p, np.float64),
+                    reference_cell]
+            call = self.lib.oracle_pressure_assemble
+        if call(*args, C.byref(n_out), C.byref(nnz), None, None, None, None) != 0:
+            raise RuntimeError("pressure assembly failed")
+        n = n_out.value
+        rp = np.empty(n + 1, np.uint32)
+        cols = np.empty(nnz.value, np.uint32)
+        vals = np.empty(nnz.value)
+        rhs = np.empty(n)
+        call(*args, C.byref(n_out), C.byref(nnz), rp.ctypes.data_as(vp), cols.ctypes.data_as(vp),
+             vals.ctypes.data_as(vp), rhs.ctypes.data_as(vp))
+        out = dict(rowp=rp, cols=cols, vals=vals, rhs=rhs)
+        if self.reference:
+            out["grad_p"] = grad
+            out["phi"] = phi_out
+        return out
+
+    def spmv(self, rp, cols, vals, x):
+        y = np.empty(len(rp) - 1)
+        self.lib.oracle_spmv(len(rp) - 1, _f(rp, np.uint32), _f(cols, np.uint32), _f(vals, np.float64),
+                             _f(x, np.float64), y)
+        return y
+
+    # -------------------------------------------------------------- AMG --
+    def amg(self, method="sa", policy="balanced", require_certificate=True, theta=0.25, prolongator_filter=0.0):
+        a = OracleAmg(self, method, policy, require_certificate)
+        getattr(self.lib, self.p + "amg_configure")(a.h, theta, prolongator_filter)
+        return a
+
+
+class OracleAmg:
+    def __init__(self, o: Oracle, method, policy, cert):
+        self.o = o
+        self.L = o.lib
+        self.p = o.p
+        m = {"sa": 1, "boomer": 0, "directcf": 0}[method]
+        pol = {"low": 0, "balanced": 1, "fast": 2}[policy]
+        self.h = getattr(self.L, self.p + "amg_create")(m, pol, int(cert))
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.L, self.p + "amg_destroy")(self.h)
+            self.h = None
+
+    def _fn(self, name):
+        return getattr(self.L, self.p + name)
+
+    def setup(self, n, rp, cols, vals):
+        self.n = n
+        return self._fn("amg_setup")(self.h, n, _f(rp, np.uint32), _f(cols, np.uint32), _f(vals, np.float64)) == 0
+
+    def update_values(self, vals):
+        return self._fn("amg_update_values")(self.h, _f(vals, np.float64)) == 0
+
+    def num_levels(self):
+        return self._fn("amg_num_levels")(self.h)
+
+    def omega(self):
+        return self._fn("amg_omega")(self.h)
+
+    def level_info(self, level):
+        n, nnz, pn = sz(), sz(), sz()
+        if self._fn("amg_level_size")(self.h, level, C.byref(n), C.byref(nnz), C.byref(pn)) != 0:
+            raise IndexError(level)
+        return n.value, nnz.value, pn.value
+
+    def level_matrix(self, level):
+        n, nnz, _ = self.level_info(level)
+        rp = np.empty(n + 1, np.uint32)
+        cols = np.empty(nnz, np.uint32)
+        vals = np.empty(nnz)
+        self._fn("amg_level_matrix")(self.h, level, rp, cols, vals)
+        return rp, cols, vals
+
+    def level_transfer(self, level):
+        n, _, pn = self.level_info(level)
+        agg = np.empty(n, np.uint64)
+        rp = np.empty(n + 1, np.uint32)
+        cols = np.empty(pn, np.uint32)
+        vals = np.empty(pn)
+        if self._fn("amg_level_transfer")(self.h, level, agg, rp, cols, vals) != 0:
+            raise IndexError(level)
+        return agg, rp, cols, vals
+
+    def apply(self, r):
+        z = np.empty(self.n)
+        if self._fn("amg_apply")(self.h, _f(r, np.float64), z) != 0:
+            return None
+        return z
+
+    def pcg(self, b, x, max_iter=1000, tol=1e-12):
+        res = OracleResult()
+        self._fn("pcg")(self.h, _f(b, np.float64), x, max_iter, tol, C.byref(res))
+        return res
+
+    def fgmres(self, b, x, restart=30, max_iter=1000, tol=1e-12):
+        res = OracleResult()
+        self._fn("fgmres")(self.h, _f(b, np.float64), x, restart, max_iter, tol, C.byref(res))
+        return res

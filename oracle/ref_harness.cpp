@@ -1,0 +1,565 @@
+// Test-only driver that exposes the reference's OWN routines through a C ABI
+// so the oracle restatement (cfdx_oracle.cpp) and the HIP product can be
+// compared with the reference itself on identical inputs.
+//
+// Built by oracle/Makefile from the reference sources where they lie
+// (-I <reference>/src, plus core/linalg/hypre_amg.cpp and core/mesh/*.cpp);
+// the output goes to oracle/_ref/libcfdxref.so. No reference source is copied
+// into this repository. Nothing in the shipped library links or loads it.
+//
+// Driven reference functions:
+//   build_fv_geometry                 physics/finite_volume_transport.h:92-175
+//   gauss_gradient_with_boundary      physics/steady_incompressible_solver.h:1634-1750
+//   make_mass_flux                    physics/steady_incompressible_solver.h:1842-1905
+//   rhie_chow_pressure_flux_internal  physics/steady_incompressible_solver.h:1967-2029
+//   pressure_face_orthogonal_coefficient / _nonorthogonal_flux  :1914-1965
+//   SparseMatrix::push_back/finalize  core/linalg/sparse_matrix.h:54-131
+//   assemble_cell_diffusion_matrix    core/solvers/scalar_diffusion.h:39-163
+//   MatrixFreeVcyclePreconditioner    core/linalg/amg_preconditioner.h (setup/apply)
+//   NativeAMGPreconditioner           core/linalg/hypre_amg.cpp:163-330
+//   solve_cg_prepared                 core/linalg/cg_solver.h:653-667
+//   solve_fgmres                      core/linalg/gmres_solver.h:678-692
+//
+// The SIMPLE corrector's matrix/RHS loop (steady_incompressible_solver.h
+// :8080-8181, :8599-8652) is inline in a 16k-line driver function and cannot
+// be called on its own; ref_pressure_assemble below drives the reference's
+// face helpers in the order that loop uses them.
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <memory>
+#include <optional>
+#include <queue>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+// The hierarchy internals (levels, transfer rows) are private members of the
+// reference class; expose them to this test driver only.
+#define private public
+#include "cfdx/core/linalg/amg_preconditioner.h"
+#undef private
+#include "cfdx/core/linalg/hypre_amg.h"
+#include "cfdx/core/linalg/cg_solver.h"
+#include "cfdx/core/linalg/gmres_solver.h"
+#include "cfdx/core/solvers/scalar_diffusion.h"
+#include "cfdx/physics/steady_incompressible_solver.h"
+
+using namespace cfdx::core;
+using cfdx::physics::FvGeometry;
+
+namespace {
+
+Mesh build_mesh(size_t np, const double* px, const double* py, const double* pz, size_t nf,
+                const uint32_t* foff, const uint32_t* fvtx, const uint32_t* owner,
+                const int32_t* neigh, size_t nc, const uint32_t* coff, const uint32_t* cfaces,
+                const uint32_t* patch_of_face, size_t n_patches, const uint8_t* patch_kind) {
+    Mesh m;
+    m.points().resize(np);
+    for (size_t i = 0; i < np; ++i) m.points().set(i, px[i], py[i], pz[i]);
+    for (size_t f = 0; f < nf; ++f) {
+        std::vector<FaceIndex> v(fvtx + foff[f], fvtx + foff[f + 1]);
+        m.faces().push_face(v);
+    }
+    m.ownership().resize(nf);
+    for (size_t f = 0; f < nf; ++f) {
+        m.ownership().set_owner(f, owner[f]);
+        m.ownership().set_neighbour(f, neigh[f] >= 0 ? std::int64_t(neigh[f]) : FaceOwnership::BOUNDARY);
+    }
+    for (size_t c = 0; c < nc; ++c) {
+        std::vector<FaceIndex> cf(cfaces + coff[c], cfaces + coff[c + 1]);
+        m.cells().push_cell(cf);
+    }
+    std::vector<std::vector<std::size_t>> pf(n_patches);
+    if (patch_of_face)
+        for (size_t f = 0; f < nf; ++f)
+            if (patch_of_face[f] < n_patches) pf[patch_of_face[f]].push_back(f);
+    for (size_t p = 0; p < n_patches; ++p) {
+        Patch pt;
+        pt.name = "patch" + std::to_string(p);
+        pt.type = patch_kind[p] == 1 ? PatchType::EMPTY
+                 : patch_kind[p] == 2 ? PatchType::SYMMETRY : PatchType::WALL;
+        pt.face_ids = pf[p];
+        m.boundary().add_patch(pt);
+    }
+    return m;
+}
+
+struct MeshArgs {
+    size_t np;
+    const double *px, *py, *pz;
+    size_t nf;
+    const uint32_t *foff, *fvtx, *owner;
+    const int32_t* neigh;
+    size_t nc;
+    const uint32_t *coff, *cfaces, *patch_of_face;
+    size_t n_patches;
+    const uint8_t* patch_kind;
+};
+
+class OwnedOperator final : public LinearOperatorBase {
+public:
+    explicit OwnedOperator(const SparseMatrix& m) : m_(m) {}
+    std::size_t rows() const noexcept override { return m_.n_rows(); }
+    std::size_t cols() const noexcept override { return m_.n_cols(); }
+    void apply(const Vector& x, Vector& y) const override {
+        const auto v = m_.matvec(x);
+        if (y.size() != v.size()) y.resize(v.size());
+        for (size_t i = 0; i < v.size(); ++i) y(i) = v[i];
+    }
+    SparseMatrix m_;
+};
+
+SparseMatrix csr_from(size_t n, const uint32_t* rowp, const uint32_t* cols, const double* vals) {
+    std::vector<uint32_t> o(rowp, rowp + n + 1), c(cols, cols + rowp[n]);
+    std::vector<double> v(vals, vals + rowp[n]);
+    return SparseMatrix::from_sorted_csr(n, n, std::move(o), std::move(c), std::move(v));
+}
+
+// Same construction arguments as NativeAMGPreconditioner::setup
+// (core/linalg/hypre_amg.cpp:172-221) for the Balanced/Low/Fast policies.
+struct RefAmg {
+    std::unique_ptr<OwnedOperator> op;
+    std::unique_ptr<MatrixFreeVcyclePreconditioner> vc;
+    std::unique_ptr<NativeAMGPreconditioner> native;
+    int method = 1, policy = 1;
+    bool cert = true;
+    double theta = 0.25, filter = 0.0;
+};
+
+}  // namespace
+
+extern "C" {
+
+struct ref_result {
+    uint32_t status;
+    uint32_t pad;
+    uint64_t iterations;
+    double residual;
+    double residual_relative;
+    uint64_t residual_replacements;
+};
+
+int ref_geometry(size_t np, const double* px, const double* py, const double* pz, size_t nf,
+                 const uint32_t* foff, const uint32_t* fvtx, const uint32_t* owner,
+                 const int32_t* neigh, size_t nc, const uint32_t* coff, const uint32_t* cfaces,
+                 double* face_centre, double* face_sf, double* cell_centre, double* cell_volume) {
+    try {
+        const uint8_t none = 0;
+        Mesh m = build_mesh(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces,
+                            nullptr, 0, &none);
+        const FvGeometry g = cfdx::physics::build_fv_geometry(m);
+        for (size_t f = 0; f < nf; ++f) {
+            face_centre[3 * f] = g.face_centres[f].x;
+            face_centre[3 * f + 1] = g.face_centres[f].y;
+            face_centre[3 * f + 2] = g.face_centres[f].z;
+            face_sf[3 * f] = g.face_area_vectors[f].x;
+            face_sf[3 * f + 1] = g.face_area_vectors[f].y;
+            face_sf[3 * f + 2] = g.face_area_vectors[f].z;
+        }
+        for (size_t c = 0; c < nc; ++c) {
+            cell_centre[3 * c] = g.cell_centres[c].x;
+            cell_centre[3 * c + 1] = g.cell_centres[c].y;
+            cell_centre[3 * c + 2] = g.cell_centres[c].z;
+            cell_volume[c] = g.cell_volumes[c];
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_geometry: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+// Geometry cache path (core/geometry/geometry_cache.h:66-226) used by the
+// scalar-diffusion assembly.
+int ref_geometry_cache(size_t np, const double* px, const double* py, const double* pz,
+                       size_t nf, const uint32_t* foff, const uint32_t* fvtx,
+                       const uint32_t* owner, const int32_t* neigh, size_t nc,
+                       const uint32_t* coff, const uint32_t* cfaces, double* face_centre,
+                       double* face_sf, double* cell_centre, double* cell_volume) {
+    try {
+        const uint8_t none = 0;
+        Mesh m = build_mesh(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces,
+                            nullptr, 0, &none);
+        const GeometryCache g = make_geometry_cache(m);
+        for (size_t f = 0; f < nf; ++f) {
+            face_centre[3 * f] = g.face_centres[f].x;
+            face_centre[3 * f + 1] = g.face_centres[f].y;
+            face_centre[3 * f + 2] = g.face_centres[f].z;
+            face_sf[3 * f] = g.face_Sf[f].x;
+            face_sf[3 * f + 1] = g.face_Sf[f].y;
+            face_sf[3 * f + 2] = g.face_Sf[f].z;
+        }
+        for (size_t c = 0; c < nc; ++c) {
+            cell_centre[3 * c] = g.cell_centres[c].x;
+            cell_centre[3 * c + 1] = g.cell_centres[c].y;
+            cell_centre[3 * c + 2] = g.cell_centres[c].z;
+            cell_volume[c] = g.cell_volumes[c];
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_geometry_cache: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+int ref_diffusion_assemble(size_t np, const double* px, const double* py, const double* pz,
+                           size_t nf, const uint32_t* foff, const uint32_t* fvtx,
+                           const uint32_t* owner, const int32_t* neigh, size_t nc,
+                           const uint32_t* coff, const uint32_t* cfaces, double gamma,
+                           const double* source, const double* face_value,
+                           const uint8_t* face_is_neumann, size_t* nnz_out, uint32_t* rowp,
+                           uint32_t* cols, double* vals, double* rhs) {
+    try {
+        const uint8_t none = 0;
+        Mesh m = build_mesh(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces,
+                            nullptr, 0, &none);
+        PoissonBoundaryCondition bc = PoissonBoundaryCondition::dirichlet(nf);
+        for (size_t f = 0; f < nf; ++f) bc.face_values[f] = face_value[f];
+        bool any_neumann = false;
+        for (size_t f = 0; f < nf; ++f) any_neumann = any_neumann || face_is_neumann[f];
+        if (any_neumann) {
+            bc.face_types.assign(nf, PoissonBoundaryType::DIRICHLET);
+            for (size_t f = 0; f < nf; ++f)
+                if (face_is_neumann[f]) bc.face_types[f] = PoissonBoundaryType::NEUMANN;
+        }
+        std::vector<double> src(source, source + nc);
+        Vector b;
+        const SparseMatrix A = assemble_cell_diffusion_matrix(m, bc, gamma, src, b);
+        *nnz_out = A.nnz();
+        if (!rowp) return 0;
+        std::copy(A.row_offsets_data(), A.row_offsets_data() + nc + 1, rowp);
+        std::copy(A.columns_data(), A.columns_data() + A.nnz(), cols);
+        std::copy(A.values_data(), A.values_data() + A.nnz(), vals);
+        for (size_t i = 0; i < nc; ++i) rhs[i] = b(i);
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_diffusion_assemble: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+// Pressure-corrector assembly driving the reference's helpers. pbc_type per
+// patch: 0 zero-gradient, 1 fixed value. rAU/U are 3 x nc SoA.
+// Returns grad_p (3 x nc SoA) computed by gauss_gradient_with_boundary.
+int ref_pressure_assemble(size_t np, const double* px, const double* py, const double* pz,
+                          size_t nf, const uint32_t* foff, const uint32_t* fvtx,
+                          const uint32_t* owner, const int32_t* neigh, size_t nc,
+                          const uint32_t* coff, const uint32_t* cfaces,
+                          const uint32_t* patch_of_face, size_t n_patches,
+                          const uint8_t* patch_kind, const uint8_t* pbc_type,
+                          const double* pbc_value, const uint8_t* vbc_type,
+                          const double* vbc_value, double rho, const double* rAU,
+                          const double* U, const double* p_in, size_t reference_cell,
+                          double* grad_out, double* phi_out, size_t* n_out, size_t* nnz_out,
+                          uint32_t* rowp, uint32_t* cols, double* vals, double* rhs) {
+    using namespace cfdx::physics;
+    try {
+        Mesh m = build_mesh(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces,
+                            patch_of_face, n_patches, patch_kind);
+        const FvGeometry g = build_fv_geometry(m);
+        ScalarBoundaryConditions pbcs;
+        VelocityBoundaryConditions vbcs;
+        for (size_t p = 0; p < n_patches; ++p) {
+            const std::string name = "patch" + std::to_string(p);
+            if (pbc_type[p] == 1) {
+                ScalarBoundaryCondition bc;
+                bc.type = ScalarBoundaryType::FIXED_VALUE;
+                bc.value = pbc_value[p];
+                pbcs[name] = bc;
+            }
+            if (vbc_type[p] == 1) {
+                VelocityBoundaryCondition bc;
+                bc.type = VelocityBoundaryCondition::Type::FIXED_VALUE;
+                bc.value = Vec3{vbc_value[3 * p], vbc_value[3 * p + 1], vbc_value[3 * p + 2]};
+                vbcs[name] = bc;
+            }
+        }
+        Field<double, Location::CELL> p(nc, "p", "Pa", 1);
+        Field<double, Location::CELL> Uf(nc, "U", "m/s", 3);
+        for (size_t c = 0; c < nc; ++c) {
+            p(c) = p_in[c];
+            Uf.component_data(0)[c] = U[c];
+            Uf.component_data(1)[c] = U[nc + c];
+            Uf.component_data(2)[c] = U[2 * nc + c];
+        }
+        std::array<std::vector<double>, 3> rau;
+        for (int d = 0; d < 3; ++d) rau[d].assign(rAU + d * nc, rAU + (d + 1) * nc);
+        const auto phi = make_mass_flux(m, g, Uf, rho, vbcs, pbcs);
+        const auto grad = gauss_gradient_with_boundary(p, m, g, pbcs);
+        for (size_t f = 0; f < nf; ++f) phi_out[f] = phi(f);
+        for (int d = 0; d < 3; ++d)
+            for (size_t c = 0; c < nc; ++c) grad_out[d * nc + c] = grad.component_data(d)[c];
+
+        std::vector<std::map<size_t, double>> rows(nc);
+        std::vector<double> diag(nc, 0.0), cont(nc, 0.0);
+        Vector b(nc, 0.0);
+        for (size_t c = 0; c < nc; ++c)
+            for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
+                const size_t f = cfaces[k];
+                const bool is_owner = m.ownership().owner(f) == c;
+                cont[c] += is_owner ? phi(f) : -phi(f);
+                if (m.ownership().neighbour(f) >= 0) {
+                    const double rc = rhie_chow_pressure_flux_internal(m, g, f, p, grad, rau, rho);
+                    cont[c] += is_owner ? -rc : rc;
+                }
+            }
+        for (size_t f = 0; f < nf; ++f) {
+            if (m.ownership().neighbour(f) < 0) continue;
+            const size_t o = m.ownership().owner(f);
+            const size_t n = size_t(m.ownership().neighbour(f));
+            const double wn = pressure_face_neighbour_weight(g, f, o, n, false);
+            const double wo = 1.0 - wn;
+            const Vec3 dAU{wo * g.cell_volumes[o] * rau[0][o] + wn * g.cell_volumes[n] * rau[0][n],
+                           wo * g.cell_volumes[o] * rau[1][o] + wn * g.cell_volumes[n] * rau[1][n],
+                           wo * g.cell_volumes[o] * rau[2][o] + wn * g.cell_volumes[n] * rau[2][n]};
+            const double coeff = rho * pressure_face_orthogonal_coefficient(
+                g.face_area_vectors[f], g.cell_centres[n] - g.cell_centres[o], dAU);
+            diag[o] += coeff;
+            diag[n] += coeff;
+            rows[o][n] -= coeff;
+            rows[n][o] -= coeff;
+        }
+        for (size_t c = 0; c < nc; ++c) {
+            rows[c][c] += diag[c];
+            b(c) = -cont[c];
+        }
+        bool fixed_p = false;
+        for (size_t f = 0; f < nf; ++f) {
+            if (m.ownership().neighbour(f) >= 0) continue;
+            const size_t patch = g.face_patch[f];
+            if (patch >= n_patches || pbc_type[patch] != 1) continue;
+            fixed_p = true;
+            const size_t o = m.ownership().owner(f);
+            const Vec3 dvec = g.face_centres[f] - g.cell_centres[o];
+            const Vec3 Sf = g.face_area_vectors[f];
+            const Vec3 dAU{g.cell_volumes[o] * rau[0][o], g.cell_volumes[o] * rau[1][o],
+                           g.cell_volumes[o] * rau[2][o]};
+            const double bc = pressure_face_orthogonal_coefficient(Sf, dvec, dAU);
+            rows[o][o] += rho * bc;
+            const Vec3 gp{grad.component_data(0)[o], grad.component_data(1)[o],
+                          grad.component_data(2)[o]};
+            const double def = pressure_face_nonorthogonal_flux(Sf, dvec, dAU, gp);
+            b(o) += rho * (bc * (pbc_value[patch] - p(o)) + def);
+        }
+        const bool eliminate = !fixed_p && nc > 1 && reference_cell < nc;
+        if (!eliminate) {
+            SparseMatrix A(nc, nc);
+            for (size_t r = 0; r < nc; ++r)
+                for (const auto& kv : rows[r]) A.push_back(r, kv.first, kv.second);
+            A.finalize();
+            *n_out = nc;
+            *nnz_out = A.nnz();
+            if (!rowp) return 0;
+            std::copy(A.row_offsets_data(), A.row_offsets_data() + nc + 1, rowp);
+            std::copy(A.columns_data(), A.columns_data() + A.nnz(), cols);
+            std::copy(A.values_data(), A.values_data() + A.nnz(), vals);
+            for (size_t i = 0; i < nc; ++i) rhs[i] = b(i);
+        } else {
+            const size_t ref = reference_cell;
+            auto ri = [ref](size_t c) { return c < ref ? c : c - 1; };
+            SparseMatrix A(nc - 1, nc - 1);
+            for (size_t r = 0; r < nc; ++r) {
+                if (r == ref) continue;
+                for (const auto& kv : rows[r]) {
+                    if (kv.first == ref) continue;
+                    A.push_back(ri(r), ri(kv.first), kv.second);
+                }
+            }
+            A.finalize();
+            *n_out = nc - 1;
+            *nnz_out = A.nnz();
+            if (!rowp) return 0;
+            std::copy(A.row_offsets_data(), A.row_offsets_data() + nc, rowp);
+            std::copy(A.columns_data(), A.columns_data() + A.nnz(), cols);
+            std::copy(A.values_data(), A.values_data() + A.nnz(), vals);
+            for (size_t r = 0; r < nc; ++r)
+                if (r != ref) rhs[ri(r)] = b(r);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_pressure_assemble: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+void* ref_amg_create(int method, int policy, int require_certificate) {
+    auto* h = new RefAmg();
+    h->method = method;
+    h->policy = policy;
+    h->cert = require_certificate != 0;
+    return h;
+}
+void ref_amg_destroy(void* p) { delete static_cast<RefAmg*>(p); }
+void ref_amg_configure(void* p, double theta, double filter) {
+    static_cast<RefAmg*>(p)->theta = theta;
+    static_cast<RefAmg*>(p)->filter = filter;
+}
+
+int ref_amg_setup(void* hp, size_t n, const uint32_t* rowp, const uint32_t* cols,
+                  const double* vals) {
+    auto* h = static_cast<RefAmg*>(hp);
+    try {
+        SparseMatrix A = csr_from(n, rowp, cols, vals);
+        double omega = 0.7;
+        size_t pre = 6, post = 6, levels = 25;
+        if (h->policy == 0) {
+            omega = 0.65;
+            pre = post = 2;
+            levels = 15;
+        } else if (h->policy == 2) {
+            omega = 0.75;
+        }
+        h->op = std::make_unique<OwnedOperator>(A);
+        h->vc = std::make_unique<MatrixFreeVcyclePreconditioner>(
+            *h->op, omega, pre, post, h->theta, levels,
+            h->method == 1 ? AMGInterpolationPolicy::SmoothedAggregation
+                           : AMGInterpolationPolicy::DirectCF,
+            false, h->cert, AMGSmootherPolicy::DampedJacobi, h->filter, 0, 1.0);
+        if (!h->vc->setup(h->op->m_)) {
+            std::cerr << "ref_amg_setup: " << h->vc->last_setup_error() << "\n";
+            return 1;
+        }
+        if (h->theta != 0.25 || h->filter != 0.0) {
+            h->native.reset();
+            return 0;
+        }
+        h->native = std::make_unique<NativeAMGPreconditioner>(
+            h->method == 1 ? NativeAMGMethod::SmoothedAggregation : NativeAMGMethod::BoomerStyle,
+            false, h->cert);
+        h->native->configure(h->policy == 0 ? AMGMemoryPolicy::Low
+                             : h->policy == 2 ? AMGMemoryPolicy::Fast
+                                              : AMGMemoryPolicy::Balanced);
+        if (!h->native->setup(A)) {
+            std::cerr << "ref_amg_setup(native): " << h->native->last_error() << "\n";
+            return 1;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_amg_setup: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+int ref_amg_update_values(void* hp, const double* vals) {
+    auto* h = static_cast<RefAmg*>(hp);
+    SparseMatrix A = h->op->m_;
+    std::copy(vals, vals + A.nnz(), A.values_data());
+    if (!h->vc->update_values(A)) return 1;
+    h->op->m_ = A;
+    return (!h->native || h->native->update_values(A)) ? 0 : 1;
+}
+
+size_t ref_amg_num_levels(void* hp) { return static_cast<RefAmg*>(hp)->vc->levels_.size(); }
+double ref_amg_omega(void* hp) { return static_cast<RefAmg*>(hp)->vc->smoother_omega(); }
+
+int ref_amg_level_size(void* hp, size_t l, size_t* n, size_t* nnz, size_t* p_nnz) {
+    auto* h = static_cast<RefAmg*>(hp);
+    if (l >= h->vc->levels_.size()) return 1;
+    const auto& L = h->vc->levels_[l];
+    *n = L.A.n_rows();
+    *nnz = L.A.nnz();
+    size_t pn = 0;
+    for (const auto& r : L.prolongation) pn += r.size();
+    *p_nnz = pn;
+    return 0;
+}
+
+int ref_amg_level_matrix(void* hp, size_t l, uint32_t* rowp, uint32_t* cols, double* vals) {
+    auto* h = static_cast<RefAmg*>(hp);
+    if (l >= h->vc->levels_.size()) return 1;
+    const auto& A = h->vc->levels_[l].A;
+    std::copy(A.row_offsets_data(), A.row_offsets_data() + A.n_rows() + 1, rowp);
+    std::copy(A.columns_data(), A.columns_data() + A.nnz(), cols);
+    std::copy(A.values_data(), A.values_data() + A.nnz(), vals);
+    return 0;
+}
+
+int ref_amg_level_transfer(void* hp, size_t l, uint64_t* aggregate, uint32_t* prowp,
+                           uint32_t* pcols, double* pvals) {
+    auto* h = static_cast<RefAmg*>(hp);
+    if (l + 1 >= h->vc->levels_.size()) return 1;
+    const auto& L = h->vc->levels_[l];
+    const size_t n = L.A.n_rows();
+    for (size_t i = 0; i < n; ++i) aggregate[i] = L.aggregate[i];
+    size_t k = 0;
+    prowp[0] = 0;
+    for (size_t i = 0; i < n; ++i) {
+        for (const auto& e : L.prolongation[i]) {
+            pcols[k] = uint32_t(e.first);
+            pvals[k] = e.second;
+            ++k;
+        }
+        prowp[i + 1] = uint32_t(k);
+    }
+    return 0;
+}
+
+int ref_amg_apply(void* hp, const double* r, double* z) {
+    auto* h = static_cast<RefAmg*>(hp);
+    const size_t n = h->op->rows();
+    Vector rv(n), zv(n);
+    for (size_t i = 0; i < n; ++i) rv(i) = r[i];
+    Preconditioner* pc = h->native ? static_cast<Preconditioner*>(h->native.get())
+                                   : static_cast<Preconditioner*>(h->vc.get());
+    if (!pc->apply(rv, zv)) return 1;
+    for (size_t i = 0; i < n; ++i) z[i] = zv(i);
+    return 0;
+}
+
+int ref_pcg(void* hp, const double* b, double* x, size_t max_iter, double tol, ref_result* res) {
+    auto* h = static_cast<RefAmg*>(hp);
+    const SparseMatrix& A = h->op->m_;
+    const size_t n = A.n_rows();
+    Vector bv(n), xv(n);
+    for (size_t i = 0; i < n; ++i) {
+        bv(i) = b[i];
+        xv(i) = x[i];
+    }
+    Preconditioner* pc = h->native ? static_cast<Preconditioner*>(h->native.get())
+                                   : static_cast<Preconditioner*>(h->vc.get());
+    const SolverResult r = solve_cg_prepared(A, bv, xv, *pc, max_iter, tol);
+    for (size_t i = 0; i < n; ++i) x[i] = xv(i);
+    res->status = uint32_t(r.status);
+    res->iterations = r.iterations;
+    res->residual = r.residual;
+    res->residual_relative = r.residual_relative;
+    res->residual_replacements = r.residual_replacements;
+    return 0;
+}
+
+int ref_fgmres(void* hp, const double* b, double* x, int restart, size_t max_iter, double tol,
+               ref_result* res) {
+    auto* h = static_cast<RefAmg*>(hp);
+    const SparseMatrix& A = h->op->m_;
+    const size_t n = A.n_rows();
+    Vector bv(n), xv(n);
+    for (size_t i = 0; i < n; ++i) {
+        bv(i) = b[i];
+        xv(i) = x[i];
+    }
+    const SolverResult r =
+        solve_fgmres(A, bv, xv, restart, max_iter, tol,
+                     h->native ? static_cast<Preconditioner*>(h->native.get())
+                               : static_cast<Preconditioner*>(h->vc.get()),
+                     PrecisionPolicy{}, false);
+    for (size_t i = 0; i < n; ++i) x[i] = xv(i);
+    res->status = uint32_t(r.status);
+    res->iterations = r.iterations;
+    res->residual = r.residual;
+    res->residual_relative = r.residual_relative;
+    res->residual_replacements = r.residual_replacements;
+    return 0;
+}
+
+}  // extern "C"

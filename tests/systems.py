@@ -1,0 +1,45 @@
+"""Seeded pressure/Poisson systems for the tests.
+
+Meshes come from libcfdxhip's generator; geometry and the assembled systems
+come from the oracle (the checker), so tests of the AMG/Krylov path receive
+exactly the matrices the reference would build.
+"""
+import numpy as np
+
+import cfdx_dev_amd as cx
+
+
+def box_mesh(nx, ny, nz, jitter=0.15, seed=1, permute=True, wall_layers=0, first_aspect=0.0):
+    return cx.generate_mesh("box", nx, ny, nz, jitter=jitter, permute=permute, seed=seed,
+                            wall_layers=wall_layers, first_aspect=first_aspect)
+
+
+def lognormal_rau(nc, sigma, seed):
+    rng = np.random.default_rng(seed)
+    return np.repeat(np.exp(sigma * rng.normal(size=nc))[None], 3, 0).ravel()
+
+
+def pressure_system(oracle, mesh, sigma=1.0, seed=1, reference_cell=0, fixed_patch=None):
+    """All-Neumann pressure Laplacian (reference cell eliminated), or with a
+    fixed-pressure patch; RHS seeded N(0,1) projected to zero mean."""
+    g = oracle.geometry(mesh)
+    nc = mesh.n_cells
+    rau = lognormal_rau(nc, sigma, seed)
+    pt = np.zeros(mesh.n_patches, np.uint8)
+    pv = np.zeros(mesh.n_patches)
+    if fixed_patch is not None:
+        pt[fixed_patch] = 1
+    p = np.zeros(nc)
+    grad = oracle.gauss_gradient(mesh, g, pt, pv, p)
+    a = oracle.pressure_assemble(mesh, g, pt, pv, 1.0, rau, np.zeros(mesh.n_faces), p, grad, reference_cell)
+    rng = np.random.default_rng(seed + 1000)
+    b = rng.normal(size=len(a["rhs"]))
+    b -= b.mean()
+    return dict(mesh=mesh, geom=g, rau=rau, n=len(a["rhs"]), rowp=a["rowp"], cols=a["cols"], vals=a["vals"], b=b)
+
+
+def csr_to_dense(rp, cols, vals, n):
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, cols[rp[i]:rp[i + 1]]] += vals[rp[i]:rp[i + 1]]
+    return A
