@@ -31,7 +31,9 @@ class AmgOptions(C.Structure):
         ("reorder", C.c_uint32),
         ("host_threads", C.c_uint32),
         ("device", C.c_int32),
-        ("reserved", C.c_uint32 * 9),
+        ("strength_threshold", C.c_double),
+        ("prolongator_filter", C.c_double),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 

@@ -80,6 +80,8 @@ void configure(cfdxhip_amg* h) {
         c.omega = 0.75;
     }
     c.require_certificate = h->opt.require_smoother_certificate != 0;
+    c.theta = h->opt.strength_threshold;
+    c.filter = h->opt.prolongator_filter;
     c.threads = h->opt.host_threads;
     h->cfg = c;
 }
@@ -157,6 +159,8 @@ void cfdxhip_amg_options_default(cfdxhip_amg_options* o) {
     o->reorder = 1;
     o->host_threads = 0;
     o->device = 0;
+    o->strength_threshold = 0.25;
+    o->prolongator_filter = 0.0;
 }
 
 cfdxhip_status cfdxhip_amg_create(const cfdxhip_amg_options* options, cfdxhip_amg** out) {
@@ -166,7 +170,9 @@ cfdxhip_status cfdxhip_amg_create(const cfdxhip_amg_options* options, cfdxhip_am
     if (!h) return CFDXHIP_ERR_OUT_OF_MEMORY;
     if (options) h->opt = *options;
     else cfdxhip_amg_options_default(&h->opt);
-    if (h->opt.method > 1 || h->opt.policy > 2 || h->opt.precision > 1) {
+    if (h->opt.method > 1 || h->opt.policy > 2 || h->opt.precision > 1 ||
+        !(h->opt.strength_threshold >= 0.0 && h->opt.strength_threshold <= 1.0) ||
+        !(h->opt.prolongator_filter >= 0.0 && h->opt.prolongator_filter < 1.0)) {
         delete h;
         return CFDXHIP_ERR_INVALID_ARGUMENT;
     }

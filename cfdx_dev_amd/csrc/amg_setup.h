@@ -46,6 +46,7 @@ struct AmgConfig {
     double omega = 0.7;        // configured damping (hypre_amg.cpp:172-186)
     uint32_t pre = 6, post = 6;
     double theta = 0.25;
+    double filter = 0.0;       // SA prolongator filter (amg_preconditioner.h:2020-2033)
     uint32_t max_levels = 25;
     bool require_certificate = true;
     uint32_t threads = 0;      // 0: hardware concurrency

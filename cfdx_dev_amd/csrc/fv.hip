@@ -27,6 +27,7 @@
 #include <stdexcept>
 
 #include "kernels.hpp"
+#include "parallel.h"
 
 namespace cfdxhip {
 
@@ -505,11 +506,14 @@ void FvMesh::create(int device, uint64_t np_, const double* hx, const double* hy
         if (h_cf[k] >= nf) throw std::invalid_argument("cell face index out of range");
     // Face-id-ascending copy of each cell's face list (global face order).
     std::vector<uint32_t> sorted(h_cf);
-    bool already = true;
-    for (uint32_t c = 0; c < nc; ++c) {
+    std::vector<uint8_t> unsorted(nc, 0);
+    parallel_for(nc, 0, [&](uint64_t c) {
         std::sort(sorted.begin() + h_coff[c], sorted.begin() + h_coff[c + 1]);
-        for (uint32_t k = h_coff[c]; k < h_coff[c + 1]; ++k) already = already && sorted[k] == h_cf[k];
-    }
+        for (uint32_t k = h_coff[c]; k < h_coff[c + 1]; ++k)
+            if (sorted[k] != h_cf[k]) unsorted[c] = 1;
+    });
+    bool already = true;
+    for (uint8_t u : unsorted) already = already && !u;
     px = upload(hx, np);
     py = upload(hy, np);
     pz = upload(hz, np);
@@ -598,37 +602,63 @@ void FvMesh::mass_flux_device(const uint8_t* d_vbt, const double* d_vbv, double 
 const PressurePlan& FvMesh::pressure_plan(bool eliminate, uint32_t ref) {
     PressurePlan& full = pressure_full;
     if (!full.rowp) {
-        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> rows(nc);  // (col, face)
-        for (uint32_t f = 0; f < nf; ++f) {
-            if (h_neigh[f] < 0) continue;
-            const uint32_t o = h_owner[f], n = uint32_t(h_neigh[f]);
-            rows[o].push_back({n, f});
-            rows[n].push_back({o, f});
-        }
-        std::vector<uint32_t> rowp(uint64_t(nc) + 1, 0), cols, rdiag, nzf_off{0}, nzf;
-        for (uint32_t c = 0; c < nc; ++c) {
-            auto& r = rows[c];
+        // Row c: {c} and the other cell of each internal face of c, sorted,
+        // duplicates (several faces joining one pair) merged with their faces
+        // listed in ascending face id. Two passes over the cell->face lists.
+        std::vector<uint32_t> cnt(uint64_t(nc) + 1, 0), fcnt(uint64_t(nc) + 1, 0);
+        auto row_entries = [&](uint32_t c, std::vector<std::pair<uint32_t, uint32_t>>& r) {
+            r.clear();
             r.push_back({c, 0xffffffffu});
-            std::sort(r.begin(), r.end());
-            for (size_t q = 0; q < r.size();) {
-                const uint32_t col = r[q].first;
-                cols.push_back(col);
-                rdiag.push_back(col == c ? c : 0xffffffffu);
-                while (q < r.size() && r[q].first == col) {
-                    if (r[q].second != 0xffffffffu) nzf.push_back(r[q].second);
-                    ++q;
-                }
-                nzf_off.push_back(uint32_t(nzf.size()));
+            for (uint32_t k = h_coff[c]; k < h_coff[c + 1]; ++k) {
+                const uint32_t f = h_cf[k];
+                if (h_neigh[f] < 0) continue;
+                const uint32_t other = h_owner[f] == c ? uint32_t(h_neigh[f]) : h_owner[f];
+                r.push_back({other, f});
             }
-            rowp[c + 1] = uint32_t(cols.size());
-            std::vector<std::pair<uint32_t, uint32_t>>().swap(r);
+            std::sort(r.begin(), r.end());
+        };
+        parallel_chunks(nc, 0, [&](uint64_t b, uint64_t e, uint32_t) {
+            std::vector<std::pair<uint32_t, uint32_t>> r;
+            for (uint64_t c = b; c < e; ++c) {
+                row_entries(uint32_t(c), r);
+                uint32_t u = 0;
+                for (size_t q = 0; q < r.size(); ++q) u += (q == 0 || r[q].first != r[q - 1].first);
+                cnt[c + 1] = u;
+                fcnt[c + 1] = uint32_t(r.size() - 1);
+            }
+        });
+        for (uint32_t c = 0; c < nc; ++c) {
+            cnt[c + 1] += cnt[c];
+            fcnt[c + 1] += fcnt[c];
         }
+        const uint64_t nnz = cnt[nc];
+        std::vector<uint32_t> rowp(cnt.begin(), cnt.end()), cols(nnz), rdiag(nnz), nzf_off(nnz + 1, 0),
+            nzf(fcnt[nc]);
+        parallel_chunks(nc, 0, [&](uint64_t b, uint64_t e, uint32_t) {
+            std::vector<std::pair<uint32_t, uint32_t>> r;
+            for (uint64_t c = b; c < e; ++c) {
+                row_entries(uint32_t(c), r);
+                uint64_t w = rowp[c];
+                uint32_t fw = fcnt[c];
+                for (size_t q = 0; q < r.size();) {
+                    const uint32_t col = r[q].first;
+                    cols[w] = col;
+                    rdiag[w] = col == c ? uint32_t(c) : 0xffffffffu;
+                    while (q < r.size() && r[q].first == col) {
+                        if (r[q].second != 0xffffffffu) nzf[fw++] = r[q].second;
+                        ++q;
+                    }
+                    nzf_off[w + 1] = fw;
+                    ++w;
+                }
+            }
+        });
         full.n = nc;
-        full.nnz = cols.size();
-        full.h_rowp = rowp;
-        full.h_cols = cols;
-        full.rowp = upload(rowp.data(), rowp.size());
-        full.cols = upload(cols.data(), cols.size());
+        full.nnz = nnz;
+        full.h_rowp = std::move(rowp);
+        full.h_cols = std::move(cols);
+        full.rowp = upload(full.h_rowp.data(), full.h_rowp.size());
+        full.cols = upload(full.h_cols.data(), full.h_cols.size());
         full.nz_row_diag = upload(rdiag.data(), rdiag.size());
         full.nzf_off = upload(nzf_off.data(), nzf_off.size());
         full.nzf = upload(nzf.data(), nzf.size());

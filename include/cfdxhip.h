@@ -87,7 +87,9 @@ typedef struct cfdxhip_amg_options {
     uint32_t host_threads;                 /* 0: all host cores for setup */
     int32_t device;                        /* HIP device ordinal; < 0: host-only handle
                                               (setup + introspection, no device work) */
-    uint32_t reserved[9];
+    double strength_threshold;             /* theta, default 0.25 (hypre_amg.cpp:203) */
+    double prolongator_filter;             /* SA P filter, default 0 (amg_preconditioner.h:2020-2033) */
+    uint32_t reserved[4];
 } cfdxhip_amg_options;
 
 typedef struct cfdxhip_amg cfdxhip_amg;
