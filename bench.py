@@ -152,14 +152,16 @@ def main():
         return r
 
     for _ in range(args.warmup):
-        run_plain()
-        run_mixed()
+        w1, w2 = run_plain(), run_mixed()
+        log(f"[bench] warmup: plain status {w1.status} {w1.iterations} it {w1.solve_seconds:.3f} s | mixed status "
+            f"{w2.status} {w2.iterations} it {w2.solve_seconds:.3f} s")
     tp, tm, rp_last, rm_last = [], [], None, None
     for k in range(args.steps):
         rp_last = run_plain()
         tp.append(rp_last.solve_seconds)
         rm_last = run_mixed()
         tm.append(rm_last.solve_seconds)
+        assert rp_last.status == 0 and rm_last.status == 0, (rp_last, rm_last, h.last_error())
         log(f"[bench] step {k}: plain PCG {rp_last.solve_seconds:.3f} s ({rp_last.iterations} it, "
             f"rel {rp_last.residual_relative:.2e}) | mixed FGMRES {rm_last.solve_seconds:.3f} s "
             f"({rm_last.iterations} it, rel {rm_last.residual_relative:.2e})")

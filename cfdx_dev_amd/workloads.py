@@ -5,7 +5,8 @@
                 Dirichlet walls (apps/verification/poisson_mms.cpp:16-25).
 * box_pressure: 3-D hex box with jittered interior vertices, randomly
                 permuted cells, optional wall-clustered layers; log-normal
-                1/a_P field with a correlation length in cells; all-Neumann
+                1/a_P field with a correlation length in cells, its median
+                set so the operator has a unit mean diagonal; all-Neumann
                 pressure system with the reference cell eliminated; RHS seeded
                 N(0,1) projected to zero mean.
 """
@@ -60,7 +61,16 @@ def box_pressure(name, dims, jitter, sigma, corr_cells, seed, wall_layers=0, fir
     rau = np.repeat(np.exp(sigma * g)[None], 3, 0).ravel()
     pt = np.zeros(mesh.n_patches, np.uint8)
     pv = np.zeros(mesh.n_patches)
-    rp, cols, vals, _ = fv.pressure_assemble(pt, pv, 1.0, rau, np.zeros(mesh.n_faces), np.zeros(nc), 0)
+    phi0, p0 = np.zeros(mesh.n_faces), np.zeros(nc)
+    rp, cols, vals, _ = fv.pressure_assemble(pt, pv, 1.0, rau, phi0, p0, 0)
+    # Median of the log-normal field: chosen so the assembled operator has a
+    # unit mean diagonal. With a median of 1 the entries are O(h^4) and the
+    # reference's absolute thresholds (coarse LU pivot 128 eps max(1, |A|),
+    # amg_preconditioner.h:2627-2629) reject a healthy coarsest level.
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    scale = 1.0 / vals[cols == rows].mean()
+    rau = rau * scale
+    rp, cols, vals, _ = fv.pressure_assemble(pt, pv, 1.0, rau, phi0, p0, 0)
     rng = np.random.default_rng(seed + 2)
     bfull = rng.standard_normal(nc)
     bfull -= bfull.mean()
