@@ -79,7 +79,7 @@ class NativeAMGPreconditioner:
     def __init__(self, method: str = "sa", policy: str = "balanced", precision: str = "fp64",
                  require_smoother_certificate: bool = True, reorder: bool = True,
                  host_threads: int = 0, device: int = 0, strength_threshold: float = 0.25,
-                 prolongator_filter: float = 0.0):
+                 prolongator_filter: float = 0.0, filter_keeps_aggregate: bool = False):
         self._lib = _native.load()
         opt = AmgOptions()
         self._lib.cfdxhip_amg_options_default(C.byref(opt))
@@ -92,6 +92,7 @@ class NativeAMGPreconditioner:
         opt.device = device
         opt.strength_threshold = strength_threshold
         opt.prolongator_filter = prolongator_filter
+        opt.filter_keeps_aggregate = int(filter_keeps_aggregate)
         h = C.c_void_p()
         st = self._lib.cfdxhip_amg_create(C.byref(opt), C.byref(h))
         if st != 0:

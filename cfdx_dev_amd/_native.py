@@ -33,7 +33,8 @@ class AmgOptions(C.Structure):
         ("device", C.c_int32),
         ("strength_threshold", C.c_double),
         ("prolongator_filter", C.c_double),
-        ("reserved", C.c_uint32 * 4),
+        ("filter_keeps_aggregate", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 

@@ -82,6 +82,7 @@ void configure(cfdxhip_amg* h) {
     c.require_certificate = h->opt.require_smoother_certificate != 0;
     c.theta = h->opt.strength_threshold;
     c.filter = h->opt.prolongator_filter;
+    c.keep_own = h->opt.filter_keeps_aggregate != 0;
     c.threads = h->opt.host_threads;
     h->cfg = c;
 }

@@ -242,8 +242,11 @@ uint32_t greedy_aggregate(uint32_t n, const std::vector<uint32_t>& noff, const s
 
 // One damped-Jacobi smoothing step of the tentative prolongator, row
 // normalised (P * 1 = 1), with the reference's drop/fallback rules.
+// keep_own (no reference counterpart): the entry of the point's own aggregate
+// survives the filter, so no aggregate can lose its whole P column (the
+// reference then rejects the hierarchy with a zero coarse diagonal).
 void smooth_tentative(const HCsr& A, const std::vector<double>& inv, const std::vector<uint32_t>& agg,
-                      uint32_t nc, uint32_t threads, double filter, HCsr& P, uint64_t& fallback) {
+                      uint32_t nc, uint32_t threads, double filter, bool keep_own, HCsr& P, uint64_t& fallback) {
     const uint32_t n = A.n;
     const double w = 2.0 / 3.0;
     std::vector<uint32_t> cnt(size_t(n) + 1, 0);
@@ -292,7 +295,9 @@ void smooth_tentative(const HCsr& A, const std::vector<double>& inv, const std::
             for (const auto& p : wts) largest = std::max(largest, std::abs(p.second / sum));
             for (const auto& p : wts) {
                 const double nv = p.second / sum;
-                if (std::isfinite(nv) && std::abs(nv) > 1e-14 && std::abs(nv) >= filter * largest) {
+                const bool own = keep_own && p.first == agg[i];
+                if (std::isfinite(nv) &&
+                    (own ? nv != 0.0 : (std::abs(nv) > 1e-14 && std::abs(nv) >= filter * largest))) {
                     oc[m] = p.first;
                     ov[m] = nv;
                     ++m;
@@ -328,8 +333,8 @@ void smooth_tentative(const HCsr& A, const std::vector<double>& inv, const std::
     for (uint8_t f : fb) fallback += f;
 }
 
-bool sa_level(const HCsr& A, const std::vector<double>& inv, double theta, double filter, uint32_t threads,
-              HCsr& P, std::vector<uint32_t>& agg, uint32_t& nc, uint64_t& fallback) {
+bool sa_level(const HCsr& A, const std::vector<double>& inv, double theta, double filter, bool keep_own,
+              uint32_t threads, HCsr& P, std::vector<uint32_t>& agg, uint32_t& nc, uint64_t& fallback) {
     std::vector<uint32_t> soff, scol, toff, tcol;
     strength_graph(A, inv, theta, threads, soff, scol);
     transpose_pattern(A.n, soff, scol, toff, tcol);
@@ -366,7 +371,7 @@ bool sa_level(const HCsr& A, const std::vector<double>& inv, double theta, doubl
         }
     });
     nc = greedy_aggregate(n, noff, ncol, agg);
-    smooth_tentative(A, inv, agg, nc, threads, filter, P, fallback);
+    smooth_tentative(A, inv, agg, nc, threads, filter, keep_own, P, fallback);
     return true;
 }
 
@@ -723,7 +728,8 @@ bool build_hierarchy(const HCsr& A, const AmgConfig& cfg, HostHierarchy& out, st
         std::vector<uint32_t> agg;
         uint32_t nc = 0;
         if (cfg.method == AMG_SMOOTHED_AGGREGATION) {
-            sa_level(L.A, L.inv_diag, cfg.theta, cfg.filter, cfg.threads, P, agg, nc, out.fallback_rows);
+            sa_level(L.A, L.inv_diag, cfg.theta, cfg.filter, cfg.keep_own, cfg.threads, P, agg, nc,
+                     out.fallback_rows);
             if (out.lv.size() == 1) {
                 std::vector<uint32_t> members(nc, 0);
                 for (uint32_t a : agg) ++members[a];

@@ -89,7 +89,11 @@ typedef struct cfdxhip_amg_options {
                                               (setup + introspection, no device work) */
     double strength_threshold;             /* theta, default 0.25 (hypre_amg.cpp:203) */
     double prolongator_filter;             /* SA P filter, default 0 (amg_preconditioner.h:2020-2033) */
-    uint32_t reserved[4];
+    uint32_t filter_keeps_aggregate;       /* 1: the filter never drops a point's own-aggregate
+                                              entry. No reference counterpart: the reference
+                                              rejects a hierarchy whose filtered P loses a
+                                              whole column (zero coarse diagonal). Default 0. */
+    uint32_t reserved[3];
 } cfdxhip_amg_options;
 
 typedef struct cfdxhip_amg cfdxhip_amg;

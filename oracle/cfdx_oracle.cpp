@@ -295,8 +295,10 @@ std::vector<std::vector<size_t>> strength_graph(const Level& L, double theta) {
 }
 
 // smooth_tentative_prolongator: core/linalg/amg_preconditioner.h:1968-2044.
+// keep_own is this build's variant (no reference counterpart): the
+// own-aggregate entry survives the filter.
 void smooth_tentative(const Level& L, const std::vector<size_t>& agg, size_t nc,
-                      std::vector<TransferRow>& P, double filter) {
+                      std::vector<TransferRow>& P, double filter, bool keep_own = false) {
     const size_t n = L.A.n;
     const double w = 2.0 / 3.0;
     P.assign(n, {});
@@ -328,7 +330,9 @@ void smooth_tentative(const Level& L, const std::vector<size_t>& agg, size_t nc,
         for (const auto& e : wts) largest = std::max(largest, std::abs(e.second / sum));
         for (const auto& e : wts) {
             const double nv = e.second / sum;
-            if (std::isfinite(nv) && std::abs(nv) > 1e-14 && std::abs(nv) >= filter * largest)
+            const bool own = keep_own && e.first == agg[i];
+            if (std::isfinite(nv) &&
+                (own ? nv != 0.0 : (std::abs(nv) > 1e-14 && std::abs(nv) >= filter * largest)))
                 P[i].push_back({e.first, nv});
         }
         double kept = 0.0;
@@ -346,7 +350,7 @@ void smooth_tentative(const Level& L, const std::vector<size_t>& agg, size_t nc,
 // core/linalg/amg_preconditioner.h:1887-1964 (symmetrised strength graph,
 // lazy max-heap of unassigned degree with lowest-index tie break).
 void sa_interpolation(const Level& L, double theta, std::vector<TransferRow>& P,
-                      std::vector<size_t>& agg, size_t& nc, double filter) {
+                      std::vector<size_t>& agg, size_t& nc, double filter, bool keep_own) {
     const size_t n = L.A.n;
     const auto S = strength_graph(L, theta);
     std::vector<std::vector<size_t>> nb = S;
@@ -390,7 +394,7 @@ void sa_interpolation(const Level& L, double theta, std::vector<TransferRow>& P,
         assign(seed, id);
         for (size_t j : nb[seed]) assign(j, id);
     }
-    smooth_tentative(L, agg, nc, P, filter);
+    smooth_tentative(L, agg, nc, P, filter, keep_own);
 }
 
 // build_cf_interpolation (Ruge-Stueben first pass with dynamic influence,
@@ -709,6 +713,7 @@ bool stable_omega(const std::vector<Level>& lv, double cfg, double& omega) {
 struct Amg {
     int method = 1;  // 0 = Boomer-style direct C/F, 1 = smoothed aggregation
     double filter = 0.0;  // prolongator filter (amg_preconditioner.h:2020-2033)
+    bool keep_own = false;  // this build's filter variant (no reference counterpart)
     double cfg_omega = 0.7, omega = 0.7;
     size_t pre = 6, post = 6, max_levels = 25;
     double theta = 0.25;
@@ -750,7 +755,7 @@ bool amg_setup(Amg& h, const Csr& A) {
         std::vector<size_t> agg;
         size_t nc = 0;
         if (h.method == 1) {
-            sa_interpolation(L, h.theta, P, agg, nc, h.filter);
+            sa_interpolation(L, h.theta, P, agg, nc, h.filter, h.keep_own);
         } else if (!cf_interpolation(L, h.theta, P, agg, nc)) {
             h.error = "Direct-CF interpolation construction failed";
             h.lv.clear();
@@ -1290,6 +1295,7 @@ void oracle_amg_configure(void* h, double theta, double filter) {
     static_cast<Amg*>(h)->theta = theta;
     static_cast<Amg*>(h)->filter = filter;
 }
+void oracle_amg_keep_own(void* h, int keep) { static_cast<Amg*>(h)->keep_own = keep != 0; }
 
 int oracle_amg_setup(void* hp, size_t n, const uint32_t* rowp, const uint32_t* cols,
                      const double* vals) {

@@ -194,9 +194,15 @@ class Oracle:
         return y
 
     # -------------------------------------------------------------- AMG --
-    def amg(self, method="sa", policy="balanced", require_certificate=True, theta=0.25, prolongator_filter=0.0):
+    def amg(self, method="sa", policy="balanced", require_certificate=True, theta=0.25, prolongator_filter=0.0,
+            filter_keeps_aggregate=False):
         a = OracleAmg(self, method, policy, require_certificate)
         getattr(self.lib, self.p + "amg_configure")(a.h, theta, prolongator_filter)
+        if filter_keeps_aggregate:
+            if self.reference:
+                raise ValueError("the reference has no filter_keeps_aggregate variant")
+            self.lib.oracle_amg_keep_own.argtypes = [vp, C.c_int]
+            self.lib.oracle_amg_keep_own(a.h, 1)
         return a
 
 
