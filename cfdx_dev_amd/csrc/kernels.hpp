@@ -92,7 +92,25 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
     const uint32_t k0 = rowp[r0], k1 = rowp[r1];
     double part = 0.0;
     if (k1 - k0 <= uint32_t(kStreamNnz)) {
-        for (uint32_t k = k0 + threadIdx.x; k < k1; k += kBlock) prod[k - k0] = vals[k] * x[cols[k]];
+        // All of this thread's matrix loads are issued before the dependent
+        // gathers of x; values and columns are read once, so they bypass the
+        // caches (non-temporal) and leave L2/MALL to x.
+        constexpr int kPer = kStreamNnz / kBlock;
+        uint32_t ci[kPer];
+        T vi[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t k = k0 + threadIdx.x + uint32_t(j) * kBlock;
+            if (k < k1) {
+                ci[j] = __builtin_nontemporal_load(cols + k);
+                vi[j] = __builtin_nontemporal_load(vals + k);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t k = k0 + threadIdx.x + uint32_t(j) * kBlock;
+            if (k < k1) prod[k - k0] = vi[j] * x[ci[j]];
+        }
         __syncthreads();
         for (uint32_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
             T s = init(row);
