@@ -78,6 +78,24 @@ static __global__ __launch_bounds__(1024) void reduce_partials_kernel(const doub
 
 // ------------------------------------------------------------------ SpMV --
 
+// s + p[b] + p[b+1] + ... + p[e-1], added strictly left to right; the LDS
+// reads of each group of eight are issued together so their latency overlaps
+// while the dependent additions keep the sequential order.
+template <class T>
+__device__ __forceinline__ T ordered_sum(const T* p, uint32_t b, uint32_t e, T s) {
+#pragma clang fp contract(off)
+    uint32_t k = b;
+    for (; k + 8 <= e; k += 8) {
+        T v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = p[k + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += v[j];
+    }
+    for (; k < e; ++k) s += p[k];
+    return s;
+}
+
 template <class T, class Init, class Epi>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __restrict__ blocks, uint32_t nb,
                                                              const uint32_t* __restrict__ rowp,
@@ -113,9 +131,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
         }
         __syncthreads();
         for (uint32_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
-            T s = init(row);
-            const uint32_t e = rowp[row + 1];
-            for (uint32_t k = rowp[row]; k < e; ++k) s += prod[k - k0];
+            const T s = ordered_sum(prod, rowp[row] - k0, rowp[row + 1] - k0, init(row));
             part += epi(row, s);
         }
     } else {
@@ -126,8 +142,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
             const uint32_t len = (k1 - base) < uint32_t(kStreamNnz) ? (k1 - base) : uint32_t(kStreamNnz);
             for (uint32_t k = threadIdx.x; k < len; k += kBlock) prod[k] = vals[base + k] * x[cols[base + k]];
             __syncthreads();
-            if (threadIdx.x == 0)
-                for (uint32_t k = 0; k < len; ++k) s += prod[k];
+            if (threadIdx.x == 0) s = ordered_sum(prod, 0u, len, s);
             __syncthreads();
         }
         if (threadIdx.x == 0) part = epi(r0, s);
