@@ -29,6 +29,7 @@ STATUS_NAMES = {0: "converged", 1: "max_iter_reached", 2: "diverged", 3: "not_ap
 METHODS = {"boomer": 0, "directcf": 0, "sa": 1, "smoothed_aggregation": 1}
 POLICIES = {"low": 0, "balanced": 1, "fast": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1}
+SMOOTHERS = {"jacobi": 0, "chebyshev": 1}
 
 
 @dataclass
@@ -79,7 +80,9 @@ class NativeAMGPreconditioner:
     def __init__(self, method: str = "sa", policy: str = "balanced", precision: str = "fp64",
                  require_smoother_certificate: bool = True, reorder: bool = True,
                  host_threads: int = 0, device: int = 0, strength_threshold: float = 0.25,
-                 prolongator_filter: float = 0.0, filter_keeps_aggregate: bool = False):
+                 prolongator_filter: float = 0.0, filter_keeps_aggregate: bool = False,
+                 smoother: str = "jacobi", smoother_fp32: str = "jacobi", chebyshev_degree: int = 3,
+                 chebyshev_ratio: float = 30.0):
         self._lib = _native.load()
         opt = AmgOptions()
         self._lib.cfdxhip_amg_options_default(C.byref(opt))
@@ -93,6 +96,10 @@ class NativeAMGPreconditioner:
         opt.strength_threshold = strength_threshold
         opt.prolongator_filter = prolongator_filter
         opt.filter_keeps_aggregate = int(filter_keeps_aggregate)
+        opt.smoother = SMOOTHERS[smoother]
+        opt.smoother_fp32 = SMOOTHERS[smoother_fp32]
+        opt.chebyshev_degree = chebyshev_degree
+        opt.chebyshev_ratio = chebyshev_ratio
         h = C.c_void_p()
         st = self._lib.cfdxhip_amg_create(C.byref(opt), C.byref(h))
         if st != 0:
@@ -175,6 +182,13 @@ class NativeAMGPreconditioner:
         if st != 0:
             raise IndexError(level)
         return agg, rp, cols, vals
+
+    def level_lambda_max(self, level: int) -> float:
+        v = C.c_double()
+        st = self._lib.cfdxhip_amg_level_lambda_max(self._h, level, C.byref(v))
+        if st != 0:
+            raise RuntimeError(f"cfdxhip_amg_level_lambda_max failed with status {st}")
+        return v.value
 
     def smoother_omega(self) -> float:
         v = C.c_double()

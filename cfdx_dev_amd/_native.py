@@ -34,7 +34,11 @@ class AmgOptions(C.Structure):
         ("strength_threshold", C.c_double),
         ("prolongator_filter", C.c_double),
         ("filter_keeps_aggregate", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("smoother", C.c_uint32),
+        ("smoother_fp32", C.c_uint32),
+        ("chebyshev_degree", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("chebyshev_ratio", C.c_double),
     ]
 
 
@@ -129,6 +133,7 @@ def load():
     _sig(lib, "cfdxhip_amg_level_matrix", st, [vp, C.c_size_t, u32p, u32p, f64p])
     _sig(lib, "cfdxhip_amg_level_transfer", st, [vp, C.c_size_t, u64p, u32p, u32p, f64p])
     _sig(lib, "cfdxhip_amg_smoother_omega", st, [vp, C.POINTER(C.c_double)])
+    _sig(lib, "cfdxhip_amg_level_lambda_max", st, [vp, C.c_size_t, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_setup_seconds", st, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_spmv_device", st, [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_mesh_generate", st, [C.POINTER(MeshSpec), C.POINTER(vp)])

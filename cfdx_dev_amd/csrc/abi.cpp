@@ -84,6 +84,10 @@ void configure(cfdxhip_amg* h) {
     c.filter = h->opt.prolongator_filter;
     c.keep_own = h->opt.filter_keeps_aggregate != 0;
     c.threads = h->opt.host_threads;
+    c.smoother64 = h->opt.smoother;
+    c.smoother32 = h->opt.smoother_fp32;
+    c.cheb_degree = h->opt.chebyshev_degree;
+    c.cheb_ratio = h->opt.chebyshev_ratio;
     h->cfg = c;
 }
 
@@ -162,6 +166,10 @@ void cfdxhip_amg_options_default(cfdxhip_amg_options* o) {
     o->device = 0;
     o->strength_threshold = 0.25;
     o->prolongator_filter = 0.0;
+    o->smoother = CFDXHIP_SMOOTHER_JACOBI;
+    o->smoother_fp32 = CFDXHIP_SMOOTHER_JACOBI;
+    o->chebyshev_degree = 3;
+    o->chebyshev_ratio = 30.0;
 }
 
 cfdxhip_status cfdxhip_amg_create(const cfdxhip_amg_options* options, cfdxhip_amg** out) {
@@ -173,7 +181,11 @@ cfdxhip_status cfdxhip_amg_create(const cfdxhip_amg_options* options, cfdxhip_am
     else cfdxhip_amg_options_default(&h->opt);
     if (h->opt.method > 1 || h->opt.policy > 2 || h->opt.precision > 1 ||
         !(h->opt.strength_threshold >= 0.0 && h->opt.strength_threshold <= 1.0) ||
-        !(h->opt.prolongator_filter >= 0.0 && h->opt.prolongator_filter < 1.0)) {
+        !(h->opt.prolongator_filter >= 0.0 && h->opt.prolongator_filter < 1.0) || h->opt.smoother > 1 ||
+        h->opt.smoother_fp32 > 1 ||
+        ((h->opt.smoother == CFDXHIP_SMOOTHER_CHEBYSHEV || h->opt.smoother_fp32 == CFDXHIP_SMOOTHER_CHEBYSHEV) &&
+         (h->opt.chebyshev_degree < 1 || h->opt.chebyshev_degree > 16 || !(h->opt.chebyshev_ratio > 1.0) ||
+          !std::isfinite(h->opt.chebyshev_ratio)))) {
         delete h;
         return CFDXHIP_ERR_INVALID_ARGUMENT;
     }
@@ -401,6 +413,14 @@ cfdxhip_status cfdxhip_amg_smoother_omega(const cfdxhip_amg* h, double* omega) {
     if (!h || !omega) return CFDXHIP_ERR_INVALID_ARGUMENT;
     if (!h->ready) return CFDXHIP_ERR_NOT_READY;
     *omega = h->host.omega;
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_level_lambda_max(const cfdxhip_amg* h, size_t level, double* lambda_max) {
+    if (!h || !lambda_max) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready || h->host.lambda_max.empty()) return CFDXHIP_ERR_NOT_READY;
+    if (level >= h->host.lambda_max.size()) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    *lambda_max = h->host.lambda_max[level];
     return CFDXHIP_OK;
 }
 

@@ -780,6 +780,9 @@ bool build_hierarchy(const HCsr& A, const AmgConfig& cfg, HostHierarchy& out, st
     } else {
         out.omega = cfg.omega;
     }
+    if (cfg.wants_lambda())
+        for (const auto& L : out.lv)
+            out.lambda_max.push_back(estimate_lambda_max(L.A, L.inv_diag, cfg.lambda_iterations, cfg.threads));
     return true;
 }
 
@@ -850,7 +853,72 @@ bool refresh_hierarchy(HostHierarchy& h, const AmgConfig& cfg, const double* val
         h.lv[l].inv_diag = std::move(newinv[l]);
     }
     h.omega = w;
+    h.lambda_max.clear();
+    if (cfg.wants_lambda())
+        for (const auto& L : h.lv)
+            h.lambda_max.push_back(estimate_lambda_max(L.A, L.inv_diag, cfg.lambda_iterations, cfg.threads));
     return true;
+}
+
+namespace {
+inline double lambda_start(uint64_t i) {
+    const uint32_t h = uint32_t(i) * 2654435761u + 0x9e3779b9u;
+    return double(h >> 8) * (1.0 / 16777216.0) - 0.5;
+}
+}  // namespace
+
+double estimate_lambda_max(const HCsr& A, const std::vector<double>& inv_diag, uint32_t iterations,
+                           uint32_t threads) {
+    const uint32_t n = A.n;
+    constexpr uint32_t kRows = 8192;
+    const uint32_t nb = (n + kRows - 1) / kRows;
+    std::vector<double> x(n), y(n), num(nb), den(nb);
+    for (uint32_t i = 0; i < n; ++i) x[i] = lambda_start(i);
+    double lambda = 0.0;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        const bool last = it + 1 == iterations;
+        parallel_for(nb, threads, [&](uint64_t b) {
+            const uint32_t r0 = uint32_t(b) * kRows, r1 = std::min(n, r0 + kRows);
+            double sn = 0.0, sd = 0.0;
+            for (uint32_t i = r0; i < r1; ++i) {
+                double s = 0.0;
+                for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) s += A.vals[k] * x[A.cols[k]];
+                y[i] = s;
+                sn += x[i] * s;
+                sd += x[i] * x[i] / inv_diag[i];
+            }
+            num[b] = sn;
+            den[b] = sd;
+        });
+        double tn = 0.0, td = 0.0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            tn += num[b];
+            td += den[b];
+        }
+        lambda = tn / td;
+        if (!last)
+            parallel_for(n, threads, [&](uint64_t i) { x[i] = inv_diag[i] * y[i]; });
+    }
+    return lambda;
+}
+
+void chebyshev_coefficients(double lambda_max, double ratio, uint32_t degree, std::vector<double>& c1,
+                            std::vector<double>& c2) {
+    const double hi = kChebUpper * lambda_max;
+    const double lo = hi / ratio;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta;
+    c1.assign(degree, 0.0);
+    c2.assign(degree, 0.0);
+    if (degree == 0) return;
+    c2[0] = 1.0 / theta;
+    double rho = 1.0 / sigma;
+    for (uint32_t k = 1; k < degree; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c1[k] = rn * rho;
+        c2[k] = 2.0 * rn / delta;
+        rho = rn;
+    }
 }
 
 bool factor_coarsest(const HCsr& A, CoarseLU& lu) {

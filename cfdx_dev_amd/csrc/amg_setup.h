@@ -40,6 +40,7 @@ struct HostLevel {
 };
 
 enum AmgMethod : uint32_t { AMG_BOOMER_STYLE = 0, AMG_SMOOTHED_AGGREGATION = 1 };
+enum AmgSmoother : uint32_t { SMOOTHER_JACOBI = 0, SMOOTHER_CHEBYSHEV = 1 };
 
 struct AmgConfig {
     uint32_t method = AMG_SMOOTHED_AGGREGATION;
@@ -51,11 +52,21 @@ struct AmgConfig {
     uint32_t max_levels = 25;
     bool require_certificate = true;
     uint32_t threads = 0;      // 0: hardware concurrency
+    // V-cycle smoother per precision: 0 damped Jacobi (the reference's),
+    // 1 Jacobi-preconditioned Chebyshev of cheb_degree steps per smoothing.
+    uint32_t smoother64 = SMOOTHER_JACOBI, smoother32 = SMOOTHER_JACOBI;
+    uint32_t cheb_degree = 3;
+    double cheb_ratio = 30.0;  // upper / lower end of the damped interval
+    uint32_t lambda_iterations = 10;
+    bool wants_lambda() const { return smoother64 == SMOOTHER_CHEBYSHEV || smoother32 == SMOOTHER_CHEBYSHEV; }
 };
 
 struct HostHierarchy {
     std::vector<HostLevel> lv;
     double omega = 0.7;
+    // Per-level estimate of lambda_max(D^-1 A); filled when a Chebyshev
+    // smoother is configured.
+    std::vector<double> lambda_max;
     uint64_t singleton_aggregates = 0;
     uint64_t fallback_rows = 0;
 };
@@ -73,6 +84,23 @@ struct CoarseLU {
     double tol = 0.0;
 };
 bool factor_coarsest(const HCsr& A, CoarseLU& lu);
+
+// Estimate of the largest eigenvalue of D^-1 A for SPD A: power iteration
+// x <- D^-1 A x from a fixed hashed start vector, returning the D-weighted
+// Rayleigh quotient x^T A x / x^T D x of the last iterate (a lower bound).
+// Reductions run over fixed 8192-row blocks summed in block order, so the
+// result does not depend on the thread count.
+double estimate_lambda_max(const HCsr& A, const std::vector<double>& inv_diag, uint32_t iterations,
+                           uint32_t threads);
+
+// Three-term Chebyshev recurrence for D^-1 A on [hi / ratio, hi] with
+// hi = kChebUpper * lambda_max (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Alg. 12.1, Jacobi-preconditioned):
+//   step 0:  d = c2[0] * D^-1 (b - A x),                x += d
+//   step k:  d = c1[k] * d + c2[k] * D^-1 (b - A x),    x += d
+constexpr double kChebUpper = 1.1;
+void chebyshev_coefficients(double lambda_max, double ratio, uint32_t degree, std::vector<double>& c1,
+                            std::vector<double>& c2);
 
 // Locality ordering of a symmetric pattern: reverse Cuthill-McKee from a
 // pseudo-peripheral vertex, per connected component. new_of_old[i] is the

@@ -145,6 +145,8 @@ void DeviceAmg::release() {
         dfree(L.x32[1]);
         dfree(L.b32);
         dfree(L.r32);
+        dfree(L.d64);
+        dfree(L.d32);
     }
     lv_.clear();
     dfree(lu_.piv);
@@ -174,6 +176,16 @@ bool DeviceAmg::build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder
     post_ = cfg.post;
     has64_ = want64;
     has32_ = want32;
+    smoother64_ = cfg.smoother64;
+    smoother32_ = cfg.smoother32;
+    cheb_degree_ = cfg.cheb_degree;
+    cheb_ratio_ = cfg.cheb_ratio;
+    if ((smoother64_ == SMOOTHER_CHEBYSHEV || smoother32_ == SMOOTHER_CHEBYSHEV) &&
+        h.lambda_max.size() != h.lv.size()) {
+        err = "Chebyshev smoother requires the eigenvalue estimates of every level";
+        return false;
+    }
+    set_chebyshev(h);
     const size_t L = h.lv.size();
     if (L == 0) {
         err = "empty hierarchy";
@@ -240,6 +252,7 @@ bool DeviceAmg::build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder
             D.x64[1] = dalloc<double>(D.n);
             D.b64 = dalloc<double>(D.n);
             D.r64 = dalloc<double>(D.n);
+            if (smoother64_ == SMOOTHER_CHEBYSHEV) D.d64 = dalloc<double>(D.n);
         }
         if (has32_) {
             std::vector<float> inv32(D.n);
@@ -249,6 +262,7 @@ bool DeviceAmg::build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder
             D.x32[1] = dalloc<float>(D.n);
             D.b32 = dalloc<float>(D.n);
             D.r32 = dalloc<float>(D.n);
+            if (smoother32_ == SMOOTHER_CHEBYSHEV) D.d32 = dalloc<float>(D.n);
         }
         HIP_OK(hipStreamSynchronize(stream_));
         if (l + 1 < L) {
@@ -325,6 +339,12 @@ bool DeviceAmg::refresh(const HostHierarchy& h, std::string& err) {
         }
     }
     omega_ = h.omega;
+    if ((smoother64_ == SMOOTHER_CHEBYSHEV || smoother32_ == SMOOTHER_CHEBYSHEV) &&
+        h.lambda_max.size() != h.lv.size()) {
+        err = "Chebyshev smoother requires the eigenvalue estimates of every level";
+        return false;
+    }
+    set_chebyshev(h);
     const HostLevel& C = h.lv.back();
     if (C.A.n <= 16) {
         CoarseLU lu;
@@ -418,6 +438,57 @@ void DeviceAmg::spmv_jacobi_sweeps(DevLevel& L, const T* b, T*& x, uint32_t swee
     }
 }
 
+void DeviceAmg::set_chebyshev(const HostHierarchy& h) {
+    cheb_c1_.assign(h.lambda_max.size(), {});
+    cheb_c2_.assign(h.lambda_max.size(), {});
+    for (size_t l = 0; l < h.lambda_max.size(); ++l)
+        chebyshev_coefficients(h.lambda_max[l], cheb_ratio_, cheb_degree_, cheb_c1_[l], cheb_c2_[l]);
+}
+
+template <class T>
+static T* d_of(DevLevel& L);
+template <>
+double* d_of<double>(DevLevel& L) {
+    return L.d64;
+}
+template <>
+float* d_of<float>(DevLevel& L) {
+    return L.d32;
+}
+
+template <class T>
+void DeviceAmg::chebyshev_sweeps(uint32_t l, const T* b, T*& x, bool from_zero) {
+    DevLevel& L = lv_[l];
+    T* const* xs = x_of<T>(L);
+    T* d = d_of<T>(L);
+    const auto& c1 = cheb_c1_[l];
+    const auto& c2 = cheb_c2_[l];
+    uint32_t k = 0;
+    if (from_zero && cheb_degree_ > 0) {
+        cheb_first_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, b, inv_of<T>(L), T(c2[0]), d, x);
+        k = 1;
+    }
+    for (; k < cheb_degree_; ++k) {
+        T* xn = (x == xs[0]) ? xs[1] : xs[0];
+        if (k == 0)
+            launch_spmv<T>(L.A, x, InitZero<T>{},
+                           EpiCheb<T, true>{b, x, inv_of<T>(L), d, xn, T(0), T(c2[0])}, stream_);
+        else
+            launch_spmv<T>(L.A, x, InitZero<T>{},
+                           EpiCheb<T, false>{b, x, inv_of<T>(L), d, xn, T(c1[k]), T(c2[k])}, stream_);
+        x = xn;
+    }
+}
+
+template <class T>
+void DeviceAmg::smooth(uint32_t l, const T* b, T*& x, uint32_t sweeps, bool from_zero) {
+    const uint32_t kind = sizeof(T) == 8 ? smoother64_ : smoother32_;
+    if (kind == SMOOTHER_CHEBYSHEV)
+        chebyshev_sweeps<T>(l, b, x, from_zero);
+    else
+        spmv_jacobi_sweeps<T>(lv_[l], b, x, sweeps, from_zero);
+}
+
 template <class T>
 bool DeviceAmg::vcycle_t(uint32_t l, const T* b, T*& x) {
     DevLevel& L = lv_[l];
@@ -432,8 +503,9 @@ bool DeviceAmg::vcycle_t(uint32_t l, const T* b, T*& x) {
         spmv_jacobi_sweeps<T>(L, b, x, 24, true);
         return true;
     }
-    if (pre_ > 0) {
-        spmv_jacobi_sweeps<T>(L, b, x, pre_, true);
+    const bool cheb = (sizeof(T) == 8 ? smoother64_ : smoother32_) == SMOOTHER_CHEBYSHEV;
+    if (cheb ? cheb_degree_ > 0 : pre_ > 0) {
+        smooth<T>(l, b, x, pre_, true);
     } else {
         fill_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, T(0), x);
     }
@@ -445,7 +517,7 @@ bool DeviceAmg::vcycle_t(uint32_t l, const T* b, T*& x) {
     T* xc = nullptr;
     if (!vcycle_t<T>(l + 1, bc, xc)) return false;
     launch_spmv<T>(L.P, xc, InitFrom<T>{x}, EpiAssign<T>{x}, stream_);
-    spmv_jacobi_sweeps<T>(L, b, x, post_, false);
+    smooth<T>(l, b, x, post_, false);
     return true;
 }
 

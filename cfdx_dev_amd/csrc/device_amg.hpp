@@ -33,6 +33,8 @@ struct DevLevel {
     float* x32[2] = {nullptr, nullptr};
     float* b32 = nullptr;
     float* r32 = nullptr;
+    double* d64 = nullptr;  // Chebyshev direction (allocated when selected)
+    float* d32 = nullptr;
     // Host mirror of the solve-ordering permutation (new index of old row).
     std::vector<uint32_t> new_of_old;
 };
@@ -96,6 +98,11 @@ private:
     bool vcycle_t(uint32_t l, const T* b, T*& x);
     template <class T>
     void spmv_jacobi_sweeps(DevLevel& L, const T* b, T*& x, uint32_t sweeps, bool from_zero);
+    template <class T>
+    void chebyshev_sweeps(uint32_t l, const T* b, T*& x, bool from_zero);
+    template <class T>
+    void smooth(uint32_t l, const T* b, T*& x, uint32_t sweeps, bool from_zero);
+    void set_chebyshev(const HostHierarchy& h);
     void release();
     bool ensure_work(uint64_t n_partials);
     void reduce(const double* partials, uint32_t n, uint32_t count, double* out);
@@ -105,6 +112,10 @@ private:
     DevCoarseLU lu_;
     double omega_ = 0.7;
     uint32_t pre_ = 6, post_ = 6;
+    uint32_t smoother64_ = SMOOTHER_JACOBI, smoother32_ = SMOOTHER_JACOBI;
+    uint32_t cheb_degree_ = 3;
+    double cheb_ratio_ = 30.0;
+    std::vector<std::vector<double>> cheb_c1_, cheb_c2_;  // per level
     bool has64_ = false, has32_ = false;
     hipStream_t stream_ = nullptr;
     uint32_t* d_perm0_ = nullptr;  // new_of_old of level 0 on device

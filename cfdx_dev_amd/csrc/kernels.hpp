@@ -224,6 +224,29 @@ struct EpiJacobi {
     __device__ void store_partial(uint32_t, double) const {}
 };
 
+// Chebyshev step (three-term recurrence, Jacobi-preconditioned):
+// dn = c1 * d + c2 * (inv_diag * (b - A x)), xn = x + dn, d <- dn.
+// kFirst: the step that starts the recurrence (no previous direction).
+template <class T, bool kFirst>
+struct EpiCheb {
+    static constexpr bool kPartial = false;
+    const T* b;
+    const T* x;
+    const T* inv_diag;
+    T* d;
+    T* xn;
+    T c1, c2;
+    __device__ double operator()(uint32_t row, T s) const {
+#pragma clang fp contract(off)
+        const T delta = inv_diag[row] * (b[row] - s);
+        const T dn = kFirst ? c2 * delta : c1 * d[row] + c2 * delta;
+        d[row] = dn;
+        xn[row] = x[row] + dn;
+        return 0.0;
+    }
+    __device__ void store_partial(uint32_t, double) const {}
+};
+
 // Prolongation x_i += sum_c P_ic xc_c accumulated onto x_i in entry order
 // (amg_preconditioner.h:2432-2436): init = x, store back.
 template <class T>
@@ -240,6 +263,20 @@ struct EpiAssign {
 // -------------------------------------------------------------- vectors --
 
 // First sweep from x = 0: A*0 = 0, so x = omega * (inv_diag * b).
+// First Chebyshev step from x = 0: d = c2 * (inv_diag * b), x = d.
+template <class T>
+__global__ __launch_bounds__(kBlock) void cheb_first_kernel(uint32_t n, const T* __restrict__ b,
+                                                           const T* __restrict__ inv_diag, T c2,
+                                                           T* __restrict__ d, T* __restrict__ x) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const T v = c2 * (inv_diag[i] * b[i]);
+        d[i] = v;
+        x[i] = v;
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void jacobi_first_kernel(uint32_t n, const T* __restrict__ b,
                                                              const T* __restrict__ inv_diag, T omega,

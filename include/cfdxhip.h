@@ -77,6 +77,15 @@ enum { CFDXHIP_AMG_BOOMER_STYLE = 0, CFDXHIP_AMG_SMOOTHED_AGGREGATION = 1 };
 enum { CFDXHIP_POLICY_LOW = 0, CFDXHIP_POLICY_BALANCED = 1, CFDXHIP_POLICY_FAST = 2 };
 /* Precision of the V-cycle (hierarchy values, smoother, transfers). */
 enum { CFDXHIP_PRECISION_FP64 = 0, CFDXHIP_PRECISION_FP32 = 1 };
+/* V-cycle smoother. JACOBI is the reference's damped Jacobi
+ * (amg_preconditioner.h:2369-2409: pre/post sweeps, certified omega).
+ * CHEBYSHEV has no counterpart inside the reference's V-cycle: the textbook
+ * Jacobi-preconditioned three-term Chebyshev recurrence of chebyshev_degree
+ * steps per smoothing on [1.1 lambda / ratio, 1.1 lambda], lambda being a
+ * 10-step power-iteration estimate of lambda_max(D^-1 A) per level. (The
+ * reference's standalone ChebyshevSmoother, chebyshev_smoother.h:11-118, uses
+ * a different update and is not on its V-cycle path.) */
+enum { CFDXHIP_SMOOTHER_JACOBI = 0, CFDXHIP_SMOOTHER_CHEBYSHEV = 1 };
 
 typedef struct cfdxhip_amg_options {
     uint32_t method;                       /* CFDXHIP_AMG_* (default smoothed aggregation) */
@@ -93,7 +102,11 @@ typedef struct cfdxhip_amg_options {
                                               entry. No reference counterpart: the reference
                                               rejects a hierarchy whose filtered P loses a
                                               whole column (zero coarse diagonal). Default 0. */
-    uint32_t reserved[3];
+    uint32_t smoother;                     /* CFDXHIP_SMOOTHER_* of FP64 V-cycles (default JACOBI) */
+    uint32_t smoother_fp32;                /* CFDXHIP_SMOOTHER_* of FP32 V-cycles (default JACOBI) */
+    uint32_t chebyshev_degree;             /* Chebyshev steps per pre/post smoothing (default 3) */
+    uint32_t reserved0;
+    double chebyshev_ratio;                /* upper / lower end of the damped interval (default 30) */
 } cfdxhip_amg_options;
 
 typedef struct cfdxhip_amg cfdxhip_amg;
@@ -139,6 +152,10 @@ cfdxhip_status cfdxhip_amg_level_matrix(const cfdxhip_amg* h, size_t level, uint
 cfdxhip_status cfdxhip_amg_level_transfer(const cfdxhip_amg* h, size_t level, uint64_t* aggregate,
                                           uint32_t* p_row_offsets, uint32_t* p_columns, double* p_values);
 cfdxhip_status cfdxhip_amg_smoother_omega(const cfdxhip_amg* h, double* omega);
+/* Power-iteration estimate of lambda_max(D^-1 A) on a level, the Chebyshev
+ * smoother's interval anchor. CFDXHIP_ERR_NOT_READY unless a Chebyshev
+ * smoother is configured. No reference counterpart. */
+cfdxhip_status cfdxhip_amg_level_lambda_max(const cfdxhip_amg* h, size_t level, double* lambda_max);
 /* Seconds spent in the last setup: host construction, device upload. */
 cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* host_seconds, double* device_seconds);
 

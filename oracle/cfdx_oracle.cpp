@@ -718,9 +718,73 @@ struct Amg {
     size_t pre = 6, post = 6, max_levels = 25;
     double theta = 0.25;
     bool require_cert = true;
+    // Smoother of this build's Chebyshev variant (no reference counterpart
+    // inside the V-cycle): 0 damped Jacobi, 1 Chebyshev.
+    int smoother = 0;
+    size_t cheb_degree = 3;
+    double cheb_ratio = 30.0;
+    std::vector<double> lambda;  // per-level lambda_max(D^-1 A) estimate
     std::vector<Level> lv;
     std::string error;
 };
+
+// Power iteration x <- D^-1 A x from the hashed start vector
+// (((i * 2654435761 + 0x9e3779b9) mod 2^32) >> 8) / 2^24 - 1/2, 10 steps,
+// returning x^T A x / x^T D x of the last iterate. Sums run over 8192-row
+// blocks (row order inside a block), then over blocks in order.
+double lambda_estimate(const Level& L, size_t iterations) {
+    const Csr& A = L.A;
+    const size_t n = A.n, rows = 8192;
+    std::vector<double> x(n), y(n);
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t hsh = uint32_t(i) * 2654435761u + 0x9e3779b9u;
+        x[i] = double(hsh >> 8) / 16777216.0 - 0.5;
+    }
+    double lambda = 0.0;
+    for (size_t it = 0; it < iterations; ++it) {
+        matvec(A, x.data(), y.data());
+        double tn = 0.0, td = 0.0;
+        for (size_t b0 = 0; b0 < n; b0 += rows) {
+            double sn = 0.0, sd = 0.0;
+            for (size_t i = b0; i < std::min(n, b0 + rows); ++i) {
+                sn += x[i] * y[i];
+                sd += x[i] * x[i] / L.inv_diag[i];
+            }
+            tn += sn;
+            td += sd;
+        }
+        lambda = tn / td;
+        for (size_t i = 0; i < n; ++i) x[i] = L.inv_diag[i] * y[i];
+    }
+    return lambda;
+}
+
+// Chebyshev three-term recurrence on [hi/ratio, hi], hi = 1.1 lambda
+// (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1):
+// rho_0 = 1/sigma, rho_k = 1/(2 sigma - rho_{k-1}); step 0 scales the
+// preconditioned residual by 1/theta, step k by 2 rho_k/delta and keeps
+// rho_k rho_{k-1} of the previous direction.
+void cheb_coeffs(double lambda, double ratio, size_t degree, std::vector<double>& keep,
+                 std::vector<double>& scale) {
+    const double hi = 1.1 * lambda, lo = hi / ratio;
+    const double theta = (hi + lo) / 2.0, delta = (hi - lo) / 2.0, sigma = theta / delta;
+    keep.assign(degree, 0.0);
+    scale.assign(degree, 0.0);
+    scale[0] = 1.0 / theta;
+    double rho = 1.0 / sigma;
+    for (size_t k = 1; k < degree; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        keep[k] = rn * rho;
+        scale[k] = 2.0 * rn / delta;
+        rho = rn;
+    }
+}
+
+void fill_lambda(Amg& h) {
+    h.lambda.clear();
+    if (h.smoother == 1)
+        for (const auto& L : h.lv) h.lambda.push_back(lambda_estimate(L, 10));
+}
 
 bool csr_valid(const Csr& A) {
     if (A.n == 0 || A.rowp.size() != A.n + 1 || A.rowp[0] != 0 || A.rowp[A.n] != A.vals.size())
@@ -791,6 +855,7 @@ bool amg_setup(Amg& h, const Csr& A) {
     } else {
         h.omega = h.cfg_omega;
     }
+    fill_lambda(h);
     return true;
 }
 
@@ -818,6 +883,25 @@ bool amg_update(Amg& h, const double* vals) {
         h.lv[l].inv_diag = std::move(staged[l].inv_diag);
     }
     h.omega = w;
+    fill_lambda(h);
+    return true;
+}
+
+// Chebyshev smoothing of degree h.cheb_degree (this build's variant).
+bool smooth_cheb(const Amg& h, size_t l, const std::vector<double>& r, std::vector<double>& x) {
+    const Level& L = h.lv[l];
+    std::vector<double> keep, scale;
+    cheb_coeffs(h.lambda[l], h.cheb_ratio, h.cheb_degree, keep, scale);
+    std::vector<double> Ax(r.size()), d(r.size(), 0.0);
+    for (size_t k = 0; k < h.cheb_degree; ++k) {
+        matvec(L.A, x.data(), Ax.data());
+        for (size_t i = 0; i < r.size(); ++i) {
+            const double delta = L.inv_diag[i] * (r[i] - Ax[i]);
+            d[i] = k == 0 ? scale[0] * delta : keep[k] * d[i] + scale[k] * delta;
+            x[i] += d[i];
+            if (!std::isfinite(x[i])) return false;
+        }
+    }
     return true;
 }
 
@@ -895,7 +979,7 @@ bool smooth_coarsest(const Amg& h, size_t l, const std::vector<double>& r, std::
 bool vcycle(const Amg& h, size_t l, const std::vector<double>& r, std::vector<double>& x) {
     if (l + 1 == h.lv.size()) return smooth_coarsest(h, l, r, x);
     const Level& L = h.lv[l];
-    if (!smooth(h, l, r, x, h.pre)) return false;
+    if (!(h.smoother == 1 ? smooth_cheb(h, l, r, x) : smooth(h, l, r, x, h.pre))) return false;
     std::vector<double> Ax(r.size());
     matvec(L.A, x.data(), Ax.data());
     const size_t nc = h.lv[l + 1].A.n;
@@ -909,7 +993,7 @@ bool vcycle(const Amg& h, size_t l, const std::vector<double>& r, std::vector<do
         for (const auto& e : L.P[i]) x[i] += e.second * cx[e.first];
         if (!std::isfinite(x[i])) return false;
     }
-    return smooth(h, l, r, x, h.post);
+    return h.smoother == 1 ? smooth_cheb(h, l, r, x) : smooth(h, l, r, x, h.post);
 }
 
 // apply: core/linalg/amg_preconditioner.h:1191-1203 (no null space).
@@ -1296,6 +1380,23 @@ void oracle_amg_configure(void* h, double theta, double filter) {
     static_cast<Amg*>(h)->filter = filter;
 }
 void oracle_amg_keep_own(void* h, int keep) { static_cast<Amg*>(h)->keep_own = keep != 0; }
+void oracle_amg_smoother(void* hp, int kind, int degree, double ratio) {
+    Amg* h = static_cast<Amg*>(hp);
+    h->smoother = kind;
+    h->cheb_degree = size_t(degree);
+    h->cheb_ratio = ratio;
+}
+double oracle_amg_lambda_max(void* hp, size_t l) {
+    Amg* h = static_cast<Amg*>(hp);
+    return l < h->lambda.size() ? h->lambda[l] : -1.0;
+}
+// Chebyshev error polynomial data for tests: the recurrence coefficients.
+void oracle_cheb_coeffs(double lambda, double ratio, size_t degree, double* keep, double* scale) {
+    std::vector<double> k, s;
+    cheb_coeffs(lambda, ratio, degree, k, s);
+    std::copy(k.begin(), k.end(), keep);
+    std::copy(s.begin(), s.end(), scale);
+}
 
 int oracle_amg_setup(void* hp, size_t n, const uint32_t* rowp, const uint32_t* cols,
                      const double* vals) {

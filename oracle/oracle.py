@@ -195,7 +195,7 @@ class Oracle:
 
     # -------------------------------------------------------------- AMG --
     def amg(self, method="sa", policy="balanced", require_certificate=True, theta=0.25, prolongator_filter=0.0,
-            filter_keeps_aggregate=False):
+            filter_keeps_aggregate=False, smoother="jacobi", chebyshev_degree=3, chebyshev_ratio=30.0):
         a = OracleAmg(self, method, policy, require_certificate)
         getattr(self.lib, self.p + "amg_configure")(a.h, theta, prolongator_filter)
         if filter_keeps_aggregate:
@@ -203,7 +203,18 @@ class Oracle:
                 raise ValueError("the reference has no filter_keeps_aggregate variant")
             self.lib.oracle_amg_keep_own.argtypes = [vp, C.c_int]
             self.lib.oracle_amg_keep_own(a.h, 1)
+        if smoother != "jacobi":
+            if self.reference:
+                raise ValueError("the reference V-cycle has no Chebyshev smoother")
+            self.lib.oracle_amg_smoother.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+            self.lib.oracle_amg_smoother(a.h, {"chebyshev": 1}[smoother], chebyshev_degree, chebyshev_ratio)
         return a
+
+    def chebyshev_coefficients(self, lam, ratio, degree):
+        keep, scale = np.empty(degree), np.empty(degree)
+        self.lib.oracle_cheb_coeffs.argtypes = [C.c_double, C.c_double, C.c_size_t, vp, vp]
+        self.lib.oracle_cheb_coeffs(lam, ratio, degree, keep.ctypes.data_as(vp), scale.ctypes.data_as(vp))
+        return keep, scale
 
 
 class OracleAmg:
@@ -236,6 +247,12 @@ class OracleAmg:
 
     def omega(self):
         return self._fn("amg_omega")(self.h)
+
+    def lambda_max(self, level):
+        f = self.L.oracle_amg_lambda_max
+        f.argtypes = [vp, C.c_size_t]
+        f.restype = C.c_double
+        return f(self.h, level)
 
     def level_info(self, level):
         n, nnz, pn = sz(), sz(), sz()

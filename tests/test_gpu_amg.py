@@ -110,3 +110,58 @@ def test_initial_guess_converged_returns_zero_iterations(oracle):
     assert r1.status == 0
     r2 = ap.solve_cg(s["b"], x, 200, 1e-10)
     assert r2.status == 0 and r2.iterations == 0
+
+
+@pytest.mark.parametrize("dims,sigma", [((12, 10, 9), 1.0), ((20, 20, 20), 1.5)])
+@pytest.mark.parametrize("degree", [2, 3])
+def test_chebyshev_vcycle_bit_identical(oracle, dims, sigma, degree):
+    """FP64 Chebyshev-smoothed V-cycle on the GPU == the oracle's, bit for bit
+    (same host eigenvalue estimates, same recurrence coefficients)."""
+    s = pressure_system(oracle, box_mesh(*dims, jitter=0.15, seed=8), sigma=sigma, seed=8)
+    ao = oracle.amg("sa", smoother="chebyshev", chebyshev_degree=degree)
+    assert ao.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    ap = cx.NativeAMGPreconditioner(smoother="chebyshev", chebyshev_degree=degree)
+    assert ap.setup(s["n"], s["rowp"], s["cols"], s["vals"]), ap.last_error()
+    for l in range(ao.num_levels()):
+        assert ap.level_lambda_max(l) == ao.lambda_max(l)
+    rng = np.random.default_rng(4)
+    for _ in range(2):
+        r = rng.normal(size=s["n"])
+        zo = ao.apply(r)
+        zp = ap.apply(r)
+        assert zp is not None, ap.last_error()
+        assert np.array_equal(zo, zp), f"max |dz| = {np.abs(zo - zp).max()}"
+
+
+def test_chebyshev_fp32_vcycle_close_to_fp64(oracle):
+    s = pressure_system(oracle, box_mesh(20, 18, 16, jitter=0.15, seed=9), sigma=1.5, seed=9)
+    ao = oracle.amg("sa", smoother="chebyshev")
+    assert ao.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    ap = cx.NativeAMGPreconditioner(precision="fp32", smoother_fp32="chebyshev")
+    assert ap.setup(s["n"], s["rowp"], s["cols"], s["vals"]), ap.last_error()
+    r = np.random.default_rng(5).normal(size=s["n"])
+    zo = ao.apply(r)
+    zp = ap.apply(r)
+    assert zp is not None, ap.last_error()
+    rel = np.linalg.norm(zp - zo) / np.linalg.norm(zo)
+    assert rel <= 1e-4, rel
+
+
+def test_mixed_fgmres_chebyshev_matches_oracle(oracle):
+    """Mixed FGMRES (FP32 Chebyshev V-cycle) reaches the FP64 true-residual
+    tolerance and the oracle's reference-smoother PCG solution."""
+    s = pressure_system(oracle, box_mesh(24, 20, 18, jitter=0.15, seed=10), sigma=1.5, seed=10)
+    ao = oracle.amg("sa")
+    assert ao.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    tol = 1e-12
+    xo = np.zeros(s["n"])
+    assert ao.pcg(s["b"], xo, 500, tol).status == 0
+    ap = cx.NativeAMGPreconditioner(precision="fp32", smoother_fp32="chebyshev")
+    assert ap.setup(s["n"], s["rowp"], s["cols"], s["vals"]), ap.last_error()
+    xp = np.zeros(s["n"])
+    rp = ap.solve_fgmres(s["b"], xp, 30, 500, tol)
+    assert rp.status == 0, ap.last_error()
+    y = oracle.spmv(s["rowp"], s["cols"], s["vals"], xp)
+    assert np.linalg.norm(s["b"] - y) / np.linalg.norm(s["b"]) <= 1.01 * tol
+    rel = np.linalg.norm(xp - xo) / np.linalg.norm(xo)
+    assert rel <= 1e-10, rel
