@@ -105,6 +105,10 @@ def main():
     ap.add_argument("--filter", type=float, default=0.2, help="SA prolongator filter")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-repeat", type=int, default=20)
+    ap.add_argument("--mixed-smoother", choices=["chebyshev", "jacobi"], default="chebyshev",
+                    help="smoother of the FP32 V-cycle inside mixed FGMRES")
+    ap.add_argument("--cheb-degree", type=int, default=3)
+    ap.add_argument("--cheb-ratio", type=float, default=30.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -126,7 +130,9 @@ def main():
     # whole P columns on this mesh and rejects the hierarchy).
     filt = args.filter
     h = cx.NativeAMGPreconditioner(method="sa", precision="fp32", prolongator_filter=filt,
-                                   filter_keeps_aggregate=True)
+                                   filter_keeps_aggregate=True, smoother="jacobi",
+                                   smoother_fp32=args.mixed_smoother, chebyshev_degree=args.cheb_degree,
+                                   chebyshev_ratio=args.cheb_ratio)
     t0 = time.perf_counter()
     ok = h.setup(s.n, s.rowp, s.cols, s.vals)
     t_setup = time.perf_counter() - t0
@@ -173,13 +179,14 @@ def main():
     nnz = len(s.vals)
     from cfdx_dev_amd._native import LIB_PATH  # noqa: F401
 
+    try:
+        hip = C.CDLL("libamdhip64.so.7")
+    except OSError:
+        hip = C.CDLL("libamdhip64.so")
+
     def spmv_time(handle, precision, dtype):
         x = np.random.default_rng(1).standard_normal(s.n).astype(dtype)
         # device buffers through the library's own spmv entry (caller-ordered)
-        try:
-            hip = C.CDLL("libamdhip64.so.7")
-        except OSError:
-            hip = C.CDLL("libamdhip64.so")
         dx, dy = C.c_void_p(), C.c_void_p()
         nb = s.n * np.dtype(dtype).itemsize
         assert hip.hipMalloc(C.byref(dx), C.c_size_t(nb)) == 0
@@ -192,6 +199,27 @@ def main():
         assert st == 0, handle.last_error()
         return sec.value
 
+    def copy_rate(nbytes, reps=10):
+        """Device-to-device copy of nbytes (runtime's copy kernel): the
+        achievable-bandwidth reference, and the known-size kernel that the
+        HBM byte counters are scaled by."""
+        da, db = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(da), C.c_size_t(nbytes)) == 0
+        assert hip.hipMalloc(C.byref(db), C.c_size_t(nbytes)) == 0
+        assert hip.hipMemset(da, 1, C.c_size_t(nbytes)) == 0
+        assert hip.hipMemcpy(db, da, C.c_size_t(nbytes), 3) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert hip.hipMemcpy(db, da, C.c_size_t(nbytes), 3) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        dt = (time.perf_counter() - t0) / reps
+        hip.hipFree(da)
+        hip.hipFree(db)
+        return 2 * nbytes / dt / 1e9
+
+    copy_gbs = copy_rate(1 << 30)
+    log(f"[bench] device copy 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
     t64 = spmv_time(plain, 0, np.float64)
     t32 = spmv_time(mixed, 1, np.float32)
     b64 = spmv_bytes(s.n, nnz, 8, 8)
@@ -225,7 +253,9 @@ def main():
             "unknowns": int(s.n),
             "nnz": int(nnz),
             "tolerance": args.tol,
-            "solver": "FGMRES(30) FP64 outer + FP32 SA V-cycle (6+6 damped Jacobi)",
+            "solver": "FGMRES(30) FP64 outer + FP32 SA V-cycle (" + (
+                f"Chebyshev degree {args.cheb_degree}, ratio {args.cheb_ratio:g}"
+                if args.mixed_smoother == "chebyshev" else "6+6 damped Jacobi") + ")",
             "plain": "PCG FP64 + FP64 SA V-cycle, true residual every iteration",
             "learned_stage": False,
             "parallelism": "single GPU",
@@ -242,6 +272,7 @@ def main():
         "amg_levels": [[int(a), int(b)] for a, b, _ in levels],
         "spmv_fp64_TBps": round(gbs64 / 1e3, 4),
         "spmv_fp32_TBps": round(gbs32 / 1e3, 4),
+        "device_copy_TBps": round(copy_gbs / 1e3, 4),
         "roofline": {
             "bound": "hbm",
             "achieved": round(gbs64, 1),

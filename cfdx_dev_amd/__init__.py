@@ -151,6 +151,17 @@ class NativeAMGPreconditioner:
             raise RuntimeError(self.last_error())
         return SolverResult.from_c(res)
 
+    def solve_fgmres_learned(self, b, x, stage: "LearnedStage", restart=30, max_iter=1000,
+                             tolerance=1e-12) -> SolverResult:
+        """Mixed FGMRES with the learned stage ahead of the FP32 V-cycle."""
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        res = Result()
+        st = self._lib.cfdxhip_amg_solve_fgmres_learned(self._h, stage._h, _f64(b), x, restart, max_iter, tolerance,
+                                                        C.byref(res))
+        if not self._ok(st):
+            raise RuntimeError(self.last_error())
+        return SolverResult.from_c(res)
+
     # ---- hierarchy introspection (reference ordering) ----
     def num_levels(self) -> int:
         v = C.c_size_t()
@@ -182,6 +193,10 @@ class NativeAMGPreconditioner:
         if st != 0:
             raise IndexError(level)
         return agg, rp, cols, vals
+
+    def set_chebyshev(self, degree: int, ratio: float = 30.0) -> bool:
+        self.last_status = self._lib.cfdxhip_amg_set_chebyshev(self._h, degree, ratio)
+        return self.last_status == 0
 
     def level_lambda_max(self, level: int) -> float:
         v = C.c_double()
@@ -360,3 +375,71 @@ class FvMesh:
                                                             C.byref(nnz), _ptr(rp), _ptr(cols), _ptr(vals),
                                                             _ptr(rhs)))
         return rp, cols, vals, rhs
+
+
+class LearnedStage:
+    """Message-passing network stage of mixed FGMRES (no reference
+    counterpart): forward, backward, Adam and training run in libcfdxhip's HIP
+    kernels on the graph of a set-up FP32 hierarchy."""
+
+    def __init__(self, device: int = 0, params=None):
+        self._lib = _native.load()
+        h = C.c_void_p()
+        st = self._lib.cfdxhip_mpnn_create(device, C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"cfdxhip_mpnn_create failed with status {st}")
+        self._h = h
+        self.nparams = int(self._lib.cfdxhip_mpnn_param_count())
+        if params is not None:
+            self.set_params(params)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.cfdxhip_mpnn_destroy(h)
+            self._h = None
+
+    def last_error(self) -> str:
+        return (self._lib.cfdxhip_mpnn_last_error(self._h) or b"").decode()
+
+    def _check(self, st):
+        if st != 0:
+            raise RuntimeError(f"status {st}: {self.last_error()}")
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self._lib.cfdxhip_mpnn_set_params(self._h, p, p.size))
+
+    def get_params(self):
+        p = np.empty(self.nparams, np.float32)
+        self._check(self._lib.cfdxhip_mpnn_get_params(self._h, p, p.size))
+        return p
+
+    def forward(self, amg: NativeAMGPreconditioner, r):
+        e = np.empty(amg.n)
+        self._check(self._lib.cfdxhip_mpnn_forward(self._h, amg._h, _f64(r), e))
+        return e
+
+    def vjp(self, amg: NativeAMGPreconditioner, r, ge):
+        g = np.empty(self.nparams, np.float32)
+        self._check(self._lib.cfdxhip_mpnn_vjp(self._h, amg._h, _f64(r), _f64(ge), g))
+        return g
+
+    def adam_step(self, grad, lr):
+        self._check(self._lib.cfdxhip_mpnn_adam_step(self._h, np.ascontiguousarray(grad, dtype=np.float32), lr))
+
+    def collect_pairs(self, amg: NativeAMGPreconditioner, b, iterations, tol=1e-10):
+        its = np.ascontiguousarray(iterations, dtype=np.uint64)
+        got = C.c_uint32()
+        res = Result()
+        self._check(self._lib.cfdxhip_mpnn_collect_pairs(self._h, amg._h, _f64(b), its, len(its), tol,
+                                                          C.byref(got), C.byref(res)))
+        return got.value, SolverResult.from_c(res)
+
+    def train(self, amg: NativeAMGPreconditioner, steps, lr):
+        losses = np.empty(steps)
+        self._check(self._lib.cfdxhip_mpnn_train(self._h, amg._h, steps, lr, losses))
+        return losses
+
+    def set_fallback(self, reference_rate, ratio=0.5, window=5):
+        self._check(self._lib.cfdxhip_mpnn_set_fallback(self._h, reference_rate, ratio, window))

@@ -19,6 +19,7 @@ u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 
@@ -134,6 +135,24 @@ def load():
     _sig(lib, "cfdxhip_amg_level_transfer", st, [vp, C.c_size_t, u64p, u32p, u32p, f64p])
     _sig(lib, "cfdxhip_amg_smoother_omega", st, [vp, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_level_lambda_max", st, [vp, C.c_size_t, C.POINTER(C.c_double)])
+    _sig(lib, "cfdxhip_amg_set_chebyshev", st, [vp, C.c_uint32, C.c_double])
+    _sig(lib, "cfdxhip_mpnn_param_count", C.c_size_t, [])
+    _sig(lib, "cfdxhip_mpnn_create", st, [C.c_int32, C.POINTER(vp)])
+    _sig(lib, "cfdxhip_mpnn_destroy", None, [vp])
+    _sig(lib, "cfdxhip_mpnn_last_error", C.c_char_p, [vp])
+    _sig(lib, "cfdxhip_mpnn_set_params", st, [vp, f32p, C.c_size_t])
+    _sig(lib, "cfdxhip_mpnn_get_params", st, [vp, f32p, C.c_size_t])
+    _sig(lib, "cfdxhip_mpnn_forward", st, [vp, vp, f64p, f64p])
+    _sig(lib, "cfdxhip_mpnn_vjp", st, [vp, vp, f64p, f64p, f32p])
+    _sig(lib, "cfdxhip_mpnn_adam_step", st, [vp, f32p, C.c_float])
+    _sig(lib, "cfdxhip_mpnn_collect_pairs", st,
+         [vp, vp, f64p, u64p, C.c_uint32, C.c_double, C.POINTER(C.c_uint32), C.POINTER(Result)])
+    _sig(lib, "cfdxhip_mpnn_train", st, [vp, vp, C.c_uint32, C.c_float, f64p])
+    _sig(lib, "cfdxhip_mpnn_set_fallback", st, [vp, C.c_double, C.c_double, C.c_uint32])
+    _sig(lib, "cfdxhip_amg_solve_fgmres_learned", st,
+         [vp, vp, f64p, f64p, C.c_int, C.c_size_t, C.c_double, C.POINTER(Result)])
+    _sig(lib, "cfdxhip_amg_solve_fgmres_learned_device", st,
+         [vp, vp, vp, vp, C.c_int, C.c_size_t, C.c_double, C.POINTER(Result)])
     _sig(lib, "cfdxhip_amg_setup_seconds", st, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_spmv_device", st, [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_mesh_generate", st, [C.POINTER(MeshSpec), C.POINTER(vp)])

@@ -21,6 +21,7 @@
 #include "device_amg.hpp"
 #include "fv.hpp"
 #include "mesh_gen.h"
+#include "mpnn.hpp"
 
 using namespace cfdxhip;
 
@@ -33,6 +34,12 @@ struct cfdxhip_amg {
     std::string error;
     double host_setup_s = 0.0, device_setup_s = 0.0;
     bool ready = false;
+};
+
+struct cfdxhip_mpnn {
+    std::unique_ptr<Mpnn> m;
+    int32_t device = 0;
+    std::string error;
 };
 
 struct cfdxhip_host_mesh {
@@ -424,6 +431,20 @@ cfdxhip_status cfdxhip_amg_level_lambda_max(const cfdxhip_amg* h, size_t level, 
     return CFDXHIP_OK;
 }
 
+cfdxhip_status cfdxhip_amg_set_chebyshev(cfdxhip_amg* h, uint32_t degree, double ratio) {
+    if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (degree < 1 || degree > 16 || !(ratio > 1.0) || !std::isfinite(ratio))
+        return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "Chebyshev degree must be in [1, 16] and ratio > 1");
+    if (!h->ready || h->host.lambda_max.empty())
+        return fail(h, CFDXHIP_ERR_NOT_READY, "no Chebyshev smoother configured on a set-up hierarchy");
+    h->opt.chebyshev_degree = degree;
+    h->opt.chebyshev_ratio = ratio;
+    h->cfg.cheb_degree = degree;
+    h->cfg.cheb_ratio = ratio;
+    if (h->dev) h->dev->set_chebyshev_params(h->host, degree, ratio);
+    return CFDXHIP_OK;
+}
+
 cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* hs, double* ds) {
     if (!h || !hs || !ds) return CFDXHIP_ERR_INVALID_ARGUMENT;
     *hs = h->host_setup_s;
@@ -677,6 +698,220 @@ cfdxhip_status cfdxhip_fv_diffusion_assemble(cfdxhip_fv* fv, double gamma, const
         hip_check(hipMemcpy(rhs, db.p, size_t(m.nc) * 8, hipMemcpyDeviceToHost), "copy");
         return CFDXHIP_OK;
     });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ learned stage --
+
+namespace {
+cfdxhip_status mfail(cfdxhip_mpnn* m, cfdxhip_status s, const std::string& msg) {
+    if (m) m->error = msg;
+    return s;
+}
+template <class F>
+cfdxhip_status mguarded(cfdxhip_mpnn* m, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return mfail(m, CFDXHIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        const std::string w = e.what();
+        return mfail(m, w.rfind("HIP error", 0) == 0 ? CFDXHIP_ERR_DEVICE : CFDXHIP_ERR_INTERNAL, w);
+    } catch (...) {
+        return mfail(m, CFDXHIP_ERR_INTERNAL, "unknown exception");
+    }
+}
+// The network runs on the finest FP32 operator of a set-up device hierarchy
+// on the same device.
+cfdxhip_status check_pair(cfdxhip_mpnn* m, const cfdxhip_amg* h) {
+    if (!m) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "no AMG handle");
+    if (!h->ready || !h->dev) return mfail(m, CFDXHIP_ERR_NOT_READY, "AMG handle has no device hierarchy");
+    if (h->opt.precision != CFDXHIP_PRECISION_FP32)
+        return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "the learned stage needs an FP32 (mixed) hierarchy");
+    if (h->opt.device != m->device) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "handles on different devices");
+    return CFDXHIP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t cfdxhip_mpnn_param_count(void) { return size_t(kMpnnParams); }
+
+cfdxhip_status cfdxhip_mpnn_create(int32_t device, cfdxhip_mpnn** out) {
+    if (!out || device < 0) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* m = new (std::nothrow) cfdxhip_mpnn();
+    if (!m) return CFDXHIP_ERR_OUT_OF_MEMORY;
+    m->device = device;
+    const cfdxhip_status s = mguarded(m, [&]() {
+        m->m = std::make_unique<Mpnn>(device);
+        return CFDXHIP_OK;
+    });
+    if (s != CFDXHIP_OK) {
+        delete m;
+        return s;
+    }
+    *out = m;
+    return CFDXHIP_OK;
+}
+
+void cfdxhip_mpnn_destroy(cfdxhip_mpnn* m) {
+    if (!m) return;
+    try {
+        (void)hipSetDevice(m->device);
+        m->m.reset();
+    } catch (...) {
+    }
+    delete m;
+}
+
+const char* cfdxhip_mpnn_last_error(const cfdxhip_mpnn* m) { return m ? m->error.c_str() : kEmpty; }
+
+cfdxhip_status cfdxhip_mpnn_set_params(cfdxhip_mpnn* m, const float* params, size_t count) {
+    if (!m || !params || count != size_t(kMpnnParams)) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "bad parameters");
+    for (size_t i = 0; i < count; ++i)
+        if (!std::isfinite(params[i])) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "non-finite parameter");
+    return mguarded(m, [&]() {
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        m->m->set_params(params);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_mpnn_get_params(cfdxhip_mpnn* m, float* params, size_t count) {
+    if (!m || !params || count != size_t(kMpnnParams)) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "bad buffer");
+    return mguarded(m, [&]() {
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        m->m->get_params(params);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_mpnn_forward(cfdxhip_mpnn* m, cfdxhip_amg* h, const double* r, double* e) {
+    const cfdxhip_status c = check_pair(m, h);
+    if (c != CFDXHIP_OK) return c;
+    if (!r || !e) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "null vector");
+    return mguarded(m, [&]() {
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        mpnn_forward_host(*m->m, *h->dev, r, e);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_mpnn_vjp(cfdxhip_mpnn* m, cfdxhip_amg* h, const double* r, const double* ge, float* grad) {
+    const cfdxhip_status c = check_pair(m, h);
+    if (c != CFDXHIP_OK) return c;
+    if (!r || !ge || !grad) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "null vector");
+    return mguarded(m, [&]() {
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        mpnn_vjp_host(*m->m, *h->dev, r, ge, grad);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_mpnn_adam_step(cfdxhip_mpnn* m, const float* grad, float lr) {
+    if (!m || !grad || !(lr > 0.0f)) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "bad Adam arguments");
+    return mguarded(m, [&]() {
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        mpnn_adam_host(*m->m, grad, lr);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_mpnn_collect_pairs(cfdxhip_mpnn* m, cfdxhip_amg* h, const double* b,
+                                          const uint64_t* iterations, uint32_t count, double tol,
+                                          uint32_t* collected, cfdxhip_result* solve) {
+    const cfdxhip_status c = check_pair(m, h);
+    if (c != CFDXHIP_OK) return c;
+    if (!b || !iterations || count == 0 || !(tol > 0.0) || !h->dev)
+        return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "bad collection arguments");
+    return mguarded(m, [&]() {
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        std::vector<uint64_t> its(iterations, iterations + count);
+        uint32_t got = 0;
+        SolveStats st;
+        std::string err;
+        const bool ok = mpnn_collect(*m->m, *h->dev, b, its, tol, got, st, err);
+        if (collected) *collected = got;
+        fill_result(st, solve);
+        if (!ok) return mfail(m, CFDXHIP_ERR_NUMERIC, err);
+        m->error.clear();
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_mpnn_train(cfdxhip_mpnn* m, cfdxhip_amg* h, uint32_t steps, float lr, double* losses) {
+    const cfdxhip_status c = check_pair(m, h);
+    if (c != CFDXHIP_OK) return c;
+    if (steps == 0 || !(lr > 0.0f)) return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "bad training arguments");
+    return mguarded(m, [&]() {
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        std::string err;
+        if (!mpnn_train(*m->m, *h->dev, steps, lr, losses, err)) return mfail(m, CFDXHIP_ERR_NOT_READY, err);
+        m->error.clear();
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_mpnn_set_fallback(cfdxhip_mpnn* m, double reference_rate, double ratio, uint32_t window) {
+    if (!m || !(reference_rate >= 0.0) || !std::isfinite(reference_rate) || !(ratio >= 0.0) || window == 0)
+        return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "bad fallback arguments");
+    m->m->reference_rate = reference_rate;
+    m->m->fallback_ratio = ratio;
+    m->m->fallback_window = window;
+    return CFDXHIP_OK;
+}
+
+static cfdxhip_status learned_common(cfdxhip_amg* h, cfdxhip_mpnn* m, const double* b, double* x, bool device,
+                                     int restart, size_t max_iter, double tol, cfdxhip_result* result) {
+    const cfdxhip_status c = check_pair(m, h);
+    if (c != CFDXHIP_OK) {
+        if (h && m) h->error = m->error;
+        return c;
+    }
+    if (!b || !x || max_iter == 0 || restart <= 0 || !(tol > 0.0) || !std::isfinite(tol))
+        return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "invalid solve arguments");
+    return guarded(h, [&]() {
+        hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        const uint32_t n = h->dev->n();
+        SolveStats st;
+        std::string err;
+        LearnedControl lc;
+        lc.stage = m->m.get();
+        lc.min_rate = m->m->fallback_ratio * m->m->reference_rate;
+        lc.window = m->m->fallback_window;
+        bool ok;
+        if (device) {
+            ok = h->dev->fgmres(b, x, restart, max_iter, tol, true, st, err, &lc);
+        } else {
+            DevBuf db(size_t(n) * 8), dx(size_t(n) * 8);
+            hip_check(hipMemcpy(db.p, b, size_t(n) * 8, hipMemcpyHostToDevice), "copy b");
+            hip_check(hipMemcpy(dx.p, x, size_t(n) * 8, hipMemcpyHostToDevice), "copy x");
+            ok = h->dev->fgmres(db.as<double>(), dx.as<double>(), restart, max_iter, tol, true, st, err, &lc);
+            hip_check(hipMemcpy(x, dx.p, size_t(n) * 8, hipMemcpyDeviceToHost), "copy x");
+        }
+        if (!ok) return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, err);
+        fill_result(st, result);
+        if (result) {
+            result->learned_stage_applications = lc.applications;
+            result->learned_stage_fallbacks = lc.fallbacks;
+        }
+        h->error.clear();
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_amg_solve_fgmres_learned(cfdxhip_amg* h, cfdxhip_mpnn* m, const double* b, double* x,
+                                                int restart, size_t max_iter, double tol, cfdxhip_result* result) {
+    return learned_common(h, m, b, x, false, restart, max_iter, tol, result);
+}
+
+cfdxhip_status cfdxhip_amg_solve_fgmres_learned_device(cfdxhip_amg* h, cfdxhip_mpnn* m, const double* b,
+                                                       double* x, int restart, size_t max_iter, double tol,
+                                                       cfdxhip_result* result) {
+    return learned_common(h, m, b, x, true, restart, max_iter, tol, result);
 }
 
 }  // extern "C"

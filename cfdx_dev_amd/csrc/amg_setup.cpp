@@ -867,39 +867,80 @@ inline double lambda_start(uint64_t i) {
 }
 }  // namespace
 
+// Largest eigenvalue of a symmetric tridiagonal matrix (diagonal a[0..m),
+// off-diagonal b[1..m)) by bisection on the Sturm sequence count.
+static double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& b) {
+    const size_t m = a.size();
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    for (size_t i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::abs(b[i]) : 0.0) + (i + 1 < m ? std::abs(b[i + 1]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    // count(x) = number of eigenvalues < x
+    auto count = [&](double x) {
+        size_t c = 0;
+        double q = 1.0;
+        for (size_t i = 0; i < m; ++i) {
+            const double bb = i > 0 ? b[i] * b[i] : 0.0;
+            q = a[i] - x - (i > 0 ? bb / q : 0.0);
+            if (q == 0.0) q = -1e-300;
+            if (q < 0.0) ++c;
+        }
+        return c;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::abs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (count(mid) >= m) hi = mid;
+        else lo = mid;
+    }
+    return hi;
+}
+
 double estimate_lambda_max(const HCsr& A, const std::vector<double>& inv_diag, uint32_t iterations,
                            uint32_t threads) {
+    // Lanczos on S = D^-1/2 A D^-1/2 (same spectrum as D^-1 A).
     const uint32_t n = A.n;
     constexpr uint32_t kRows = 8192;
     const uint32_t nb = (n + kRows - 1) / kRows;
-    std::vector<double> x(n), y(n), num(nb), den(nb);
-    for (uint32_t i = 0; i < n; ++i) x[i] = lambda_start(i);
-    double lambda = 0.0;
-    for (uint32_t it = 0; it < iterations; ++it) {
-        const bool last = it + 1 == iterations;
+    std::vector<double> sq(n), v(n), vp(n, 0.0), w(n), part(nb);
+    for (uint32_t i = 0; i < n; ++i) sq[i] = std::sqrt(inv_diag[i]);
+    auto block_sum = [&](auto&& f) {
         parallel_for(nb, threads, [&](uint64_t b) {
             const uint32_t r0 = uint32_t(b) * kRows, r1 = std::min(n, r0 + kRows);
-            double sn = 0.0, sd = 0.0;
-            for (uint32_t i = r0; i < r1; ++i) {
-                double s = 0.0;
-                for (uint32_t k = A.rowp[i]; k < A.rowp[i + 1]; ++k) s += A.vals[k] * x[A.cols[k]];
-                y[i] = s;
-                sn += x[i] * s;
-                sd += x[i] * x[i] / inv_diag[i];
-            }
-            num[b] = sn;
-            den[b] = sd;
+            double acc = 0.0;
+            for (uint32_t i = r0; i < r1; ++i) acc += f(i);
+            part[b] = acc;
         });
-        double tn = 0.0, td = 0.0;
-        for (uint32_t b = 0; b < nb; ++b) {
-            tn += num[b];
-            td += den[b];
-        }
-        lambda = tn / td;
-        if (!last)
-            parallel_for(n, threads, [&](uint64_t i) { x[i] = inv_diag[i] * y[i]; });
+        double t = 0.0;
+        for (uint32_t b = 0; b < nb; ++b) t += part[b];
+        return t;
+    };
+    for (uint32_t i = 0; i < n; ++i) v[i] = lambda_start(i);
+    double nv = std::sqrt(block_sum([&](uint32_t i) { return v[i] * v[i]; }));
+    for (uint32_t i = 0; i < n; ++i) v[i] /= nv;
+    std::vector<double> alpha, beta(1, 0.0);
+    const uint32_t steps = std::min<uint32_t>(iterations, n);
+    for (uint32_t k = 0; k < steps; ++k) {
+        parallel_for(n, threads, [&](uint64_t i) {
+            double acc = 0.0;
+            for (uint32_t q = A.rowp[i]; q < A.rowp[i + 1]; ++q) acc += A.vals[q] * (sq[A.cols[q]] * v[A.cols[q]]);
+            w[i] = sq[i] * acc;
+        });
+        const double a = block_sum([&](uint32_t i) { return w[i] * v[i]; });
+        alpha.push_back(a);
+        const double bprev = beta.back();
+        parallel_for(n, threads, [&](uint64_t i) { w[i] = (w[i] - a * v[i]) - bprev * vp[i]; });
+        const double bn = std::sqrt(block_sum([&](uint32_t i) { return w[i] * w[i]; }));
+        if (!(bn > 1e-14 * std::abs(a)) || k + 1 == steps) break;
+        beta.push_back(bn);
+        parallel_for(n, threads, [&](uint64_t i) {
+            vp[i] = v[i];
+            v[i] = w[i] / bn;
+        });
     }
-    return lambda;
+    beta.resize(alpha.size());
+    return tridiag_max_eig(alpha, beta);
 }
 
 void chebyshev_coefficients(double lambda_max, double ratio, uint32_t degree, std::vector<double>& c1,

@@ -57,7 +57,7 @@ struct AmgConfig {
     uint32_t smoother64 = SMOOTHER_JACOBI, smoother32 = SMOOTHER_JACOBI;
     uint32_t cheb_degree = 3;
     double cheb_ratio = 30.0;  // upper / lower end of the damped interval
-    uint32_t lambda_iterations = 10;
+    uint32_t lambda_iterations = 20;
     bool wants_lambda() const { return smoother64 == SMOOTHER_CHEBYSHEV || smoother32 == SMOOTHER_CHEBYSHEV; }
 };
 
@@ -85,11 +85,12 @@ struct CoarseLU {
 };
 bool factor_coarsest(const HCsr& A, CoarseLU& lu);
 
-// Estimate of the largest eigenvalue of D^-1 A for SPD A: power iteration
-// x <- D^-1 A x from a fixed hashed start vector, returning the D-weighted
-// Rayleigh quotient x^T A x / x^T D x of the last iterate (a lower bound).
-// Reductions run over fixed 8192-row blocks summed in block order, so the
-// result does not depend on the thread count.
+// Estimate of the largest eigenvalue of D^-1 A for SPD A: `iterations`
+// Lanczos steps on D^-1/2 A D^-1/2 from a fixed hashed start vector; the
+// largest Ritz value (a lower bound, accurate for the extreme eigenvalue) is
+// found by bisection on the tridiagonal's Sturm count. Reductions run over
+// fixed 8192-row blocks summed in block order, so the result does not depend
+// on the thread count.
 double estimate_lambda_max(const HCsr& A, const std::vector<double>& inv_diag, uint32_t iterations,
                            uint32_t threads);
 

@@ -600,11 +600,13 @@ bool DeviceAmg::pcg(const double* b_caller, double* x_caller, uint64_t max_iter,
         st.iterations = it;
         st.residual = res;
         st.residual_relative = ref > 0.0 ? res / ref : 0.0;
+        if (on_iterate_) on_iterate_(~uint64_t(0), x);
         gather_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x, x_caller);
         HIP_OK(hipStreamSynchronize(stream_));
         st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return true;
     };
+    if (on_iterate_) on_iterate_(0, x);
     auto true_residual = [&]() {
         launch_spmv<double>(L0.A, x, InitZero<double>{}, EpiResidual<double, true>{b, rv, P1}, stream_);
         reduce(P1, nbA, 1, scal_ + 5);
@@ -632,6 +634,7 @@ bool DeviceAmg::pcg(const double* b_caller, double* x_caller, uint64_t max_iter,
         launch_spmv<double>(L0.A, p, InitZero<double>{}, EpiStoreDot<double>{Ap, p, P0}, stream_);
         reduce(P0, nbA, 1, scal_ + 2);
         cg_update_kernel<<<gs, kBlock, 0, stream_>>>(n, rsold, scal_ + 2, p, Ap, x, r, P2);
+        if (on_iterate_) on_iterate_(it, x);
         if (!precondition()) {
             const double tr = true_residual();
             return finish(3, it, tr);
@@ -787,8 +790,102 @@ __global__ __launch_bounds__(kBlock) void basis_axpy_kernel(uint32_t n, uint32_t
 
 }  // namespace
 
+Level0View DeviceAmg::level0() const {
+    Level0View v;
+    if (lv_.empty()) return v;
+    v.n = lv_[0].n;
+    v.rowp = lv_[0].A.rowp;
+    v.cols = lv_[0].A.cols;
+    v.vals = lv_[0].A.v32;
+    v.inv_diag = lv_[0].inv32;
+    return v;
+}
+
+void DeviceAmg::spmv32(const float* x, float* y) {
+    launch_spmv<float>(lv_[0].A, x, InitZero<float>{}, EpiStore<float>{y}, stream_);
+}
+
+void DeviceAmg::residual32(const float* b, const float* x, float* r) {
+    launch_spmv<float>(lv_[0].A, x, InitZero<float>{}, EpiResidual<float, false>{b, r, nullptr}, stream_);
+}
+
+namespace {
+// e = xf - xk, r = b - A xk (r given), both to FP32.
+__global__ __launch_bounds__(kBlock) void pair_kernel(uint32_t n, const double* __restrict__ xf,
+                                                     const double* __restrict__ xk, const double* __restrict__ r,
+                                                     float* __restrict__ rout, float* __restrict__ eout) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        eout[i] = float(xf[i] - xk[i]);
+        rout[i] = float(r[i]);
+    }
+}
+}  // namespace
+
+bool DeviceAmg::collect_pairs(const double* b_caller, const std::vector<uint64_t>& iterations, double tol,
+                              float* r_out, float* e_out, uint32_t& got, SolveStats& st, std::string& err) {
+    got = 0;
+    if (!has64_ || lv_.empty()) {
+        err = "pair collection requires the FP64 hierarchy";
+        return false;
+    }
+    const uint32_t n = lv_[0].n;
+    const size_t K = iterations.size();
+    std::vector<double*> rec(K, nullptr);
+    for (auto& p : rec) p = dalloc<double>(n);
+    double* xf = dalloc<double>(n);
+    double* xc = dalloc<double>(n);
+    std::vector<uint8_t> have(K, 0);
+    on_iterate_ = [&](uint64_t it, const double* xs) {
+        if (it == ~uint64_t(0)) {
+            HIP_OK(hipMemcpyAsync(xf, xs, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+            return;
+        }
+        for (size_t k = 0; k < K; ++k)
+            if (iterations[k] == it) {
+                HIP_OK(hipMemcpyAsync(rec[k], xs, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
+                have[k] = 1;
+            }
+    };
+    bool ok = false;
+    try {
+        HIP_OK(hipMemsetAsync(xc, 0, size_t(n) * 8, stream_));
+        ok = pcg(b_caller, xc, 100000, tol, st, err);
+    } catch (...) {
+        on_iterate_ = nullptr;
+        for (auto& p : rec) dfree(p);
+        dfree(xf);
+        dfree(xc);
+        throw;
+    }
+    on_iterate_ = nullptr;
+    if (ok && st.status == 0) {
+        // b in the solve ordering into xc, then r_k = b - A x_k.
+        double* bs = xc;
+        scatter_perm_kernel<double, double><<<grid_for(n), kBlock, 0, stream_>>>(n, d_perm0_, b_caller, bs);
+        double* rk = need_vec(work_, 2, n);
+        for (size_t k = 0; k < K; ++k) {
+            if (!have[k]) continue;
+            launch_spmv<double>(lv_[0].A, rec[k], InitZero<double>{}, EpiResidual<double, false>{bs, rk, nullptr},
+                                stream_);
+            pair_kernel<<<grid_for(n), kBlock, 0, stream_>>>(n, xf, rec[k], rk, r_out + size_t(got) * n,
+                                                             e_out + size_t(got) * n);
+            ++got;
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+    } else if (ok) {
+        err = "pair collection: PCG did not converge (status " + std::to_string(st.status) + ")";
+        ok = false;
+    }
+    for (auto& p : rec) dfree(p);
+    dfree(xf);
+    dfree(xc);
+    return ok;
+}
+
 bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, uint64_t max_iter, double tol,
-                       bool mixed, SolveStats& st, std::string& err) {
+                       bool mixed, SolveStats& st, std::string& err, LearnedControl* learned) {
     const auto t0 = std::chrono::steady_clock::now();
     if (!has64_ && !mixed) {
         err = "FP64 FGMRES preconditioner requires the FP64 hierarchy";
@@ -861,17 +958,40 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
         fetch_scalars(1);
         return std::sqrt(h_scal_[0]);
     };
+    if (learned && (!mixed || !learned->stage)) {
+        err = "the learned stage runs inside mixed-precision FGMRES only";
+        return false;
+    }
+    bool stage_on = learned != nullptr;
+    std::vector<double> hist;  // residual estimate after each iteration (hist[0] = initial)
     auto precondition = [&](const double* v, double* z) {
         ++st.pc_applications;
         if (!mixed) return vcycle64(v, z);
         convert_kernel<double, float><<<g, kBlock, 0, stream_>>>(n, v, r32);
-        if (!vcycle32(r32, z32)) return false;
+        if (stage_on) {
+            ++learned->applications;
+            if (!learned->stage->apply(*this, r32, z32, err)) return false;
+        } else if (!vcycle32(r32, z32)) {
+            return false;
+        }
         convert_kernel<float, double><<<g, kBlock, 0, stream_>>>(n, z32, z);
         return true;
+    };
+    auto monitor = [&](double est) {
+        hist.push_back(est);
+        if (!stage_on || hist.size() <= learned->window) return;
+        const size_t k = hist.size() - 1;
+        const double a = hist[k - learned->window], b2 = hist[k];
+        const double rate = (a > 0.0 && b2 > 0.0) ? std::log10(a / b2) / double(learned->window) : 0.0;
+        if (rate < learned->min_rate) {
+            stage_on = false;
+            ++learned->fallbacks;
+        }
     };
     double beta = true_res();
     if (!std::isfinite(beta)) return finish(2, 0, beta);
     if (beta <= tolv) return finish(0, 0, beta);
+    hist.push_back(beta);
     uint64_t its = 0;
     double prev_cycle = beta, best = beta;
     HIP_OK(hipMemcpyAsync(bestx, x, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
@@ -957,6 +1077,7 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
             ++its;
             ++used;
             est = std::abs(gv[j + 1]);
+            monitor(est);
             if (bd) {
                 brk = true;
                 break;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,38 @@ void gather_d2d(uint32_t n, const uint32_t* perm, const double* src, double* dst
 void scatter_d2f(uint32_t n, const uint32_t* perm, const double* src, float* dst, hipStream_t s);
 void gather_f2d(uint32_t n, const uint32_t* perm, const float* src, double* dst, hipStream_t s);
 
+class DeviceAmg;
+
+// A preconditioner stage run ahead of (and around) the FP32 V-cycle inside
+// mixed FGMRES: z = stage(r) on FP32 vectors in the solve ordering.
+class PreconditionerStage {
+public:
+    virtual ~PreconditionerStage() = default;
+    virtual bool apply(DeviceAmg& amg, const float* r, float* z, std::string& err) = 0;
+};
+
+// Learned-stage control of one mixed FGMRES solve. The stage is dropped for
+// the rest of the solve (plain V-cycle, counted as a fallback) once the
+// per-iteration residual reduction over the last `window` iterations,
+// log10(|g_{k-window}| / |g_k|) / window from the Givens estimates, drops
+// below min_rate.
+struct LearnedControl {
+    PreconditionerStage* stage = nullptr;
+    double min_rate = 0.0;
+    uint32_t window = 5;
+    uint64_t applications = 0;
+    uint64_t fallbacks = 0;
+};
+
+// Finest-level operator as the learned stage sees it (solve ordering, FP32).
+struct Level0View {
+    uint32_t n = 0;
+    const uint32_t* rowp = nullptr;
+    const uint32_t* cols = nullptr;
+    const float* vals = nullptr;
+    const float* inv_diag = nullptr;
+};
+
 class DeviceAmg {
 public:
     DeviceAmg() = default;
@@ -86,9 +119,28 @@ public:
     // Krylov solvers on caller-ordered device vectors (FP64).
     bool pcg(const double* b, double* x, uint64_t max_iter, double tol, SolveStats& st, std::string& err);
     bool fgmres(const double* b, double* x, int restart, uint64_t max_iter, double tol, bool mixed,
-                SolveStats& st, std::string& err);
+                SolveStats& st, std::string& err, LearnedControl* learned = nullptr);
+    // Runs FP64 PCG from x = 0 to tol on b (caller order, device) and returns
+    // (residual, error) pairs at the requested iterations: r_k = b - A x_k,
+    // e_k = x_final - x_k, FP32, solve ordering, pair k at r_out + k n.
+    bool collect_pairs(const double* b, const std::vector<uint64_t>& iterations, double tol, float* r_out,
+                       float* e_out, uint32_t& got, SolveStats& st, std::string& err);
+    Level0View level0() const;
+    // y = A x on the finest level, FP32, solve ordering.
+    void spmv32(const float* x, float* y);
+    // r = b - A x on the finest level, FP32, solve ordering.
+    void residual32(const float* b, const float* x, float* r);
+    const uint32_t* device_perm0() const { return d_perm0_; }
     // y = A x on caller-ordered device vectors; returns seconds per launch.
     bool spmv_caller(uint32_t precision, const void* x, void* y, uint32_t repeat, double& sec, std::string& err);
+
+    // Changes the Chebyshev degree / interval ratio of a built hierarchy
+    // (coefficients only; the eigenvalue estimates are kept).
+    void set_chebyshev_params(const HostHierarchy& h, uint32_t degree, double ratio) {
+        cheb_degree_ = degree;
+        cheb_ratio_ = ratio;
+        set_chebyshev(h);
+    }
 
     hipStream_t stream() const { return stream_; }
     const std::vector<uint32_t>& perm0() const { return lv_[0].new_of_old; }
@@ -117,6 +169,10 @@ private:
     double cheb_ratio_ = 30.0;
     std::vector<std::vector<double>> cheb_c1_, cheb_c2_;  // per level
     bool has64_ = false, has32_ = false;
+    // Called by pcg with (iteration, x in the solve ordering) after every
+    // update of x, with iteration 0 for the initial guess and UINT64_MAX for
+    // the returned iterate.
+    std::function<void(uint64_t, const double*)> on_iterate_;
     hipStream_t stream_ = nullptr;
     uint32_t* d_perm0_ = nullptr;  // new_of_old of level 0 on device
     // Krylov work (caller size n0), FP64.

@@ -1,0 +1,710 @@
+// Learned preconditioner stage: forward, backward and Adam kernels (HIP,
+// gfx950). Layout and algorithm are described in mpnn.hpp.
+//
+// Tiling: every node kernel works on tiles of 64 nodes per 256-thread
+// workgroup. The gather phase gives each node 32 lanes (one per channel); the
+// dense phase gives each wavefront 16 nodes and runs 16x16x16 BF16 MFMAs on
+// LDS tiles. Parameter gradients are written as per-tile partials (one row of
+// kMpnnParams floats per tile) and summed in tile order: no atomics, the
+// result does not depend on scheduling.
+#include "mpnn.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "kernels.hpp"
+
+namespace cfdxhip {
+
+#define MPNN_HIP(expr)                                                                            \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                  \
+    } while (0)
+
+namespace {
+
+constexpr int kTile = 64;   // nodes per workgroup
+constexpr int kLd = 68;     // LDS row stride (elements) of 64-wide BF16 tiles
+constexpr int kLd32 = 36;   // LDS row stride of 32-wide BF16 tiles
+constexpr int kMaxPartBlocks = 4096;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+T* mpnn_alloc(uint64_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    MPNN_HIP(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+template <class T>
+void mpnn_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+// D (16x16, FP32) += A (16 x K) * B (K x 16), A row-major in LDS with stride
+// lda, B supplied transposed (Bt[n][k], stride ldb). Lane l feeds
+// A[l%16][k0 + 4(l/16) + 0..3] and Bt[l%16][k0 + 4(l/16) + 0..3] per K step of
+// 16; D comes back as D[4(l/16) + j][l%16], j = 0..3.
+__device__ __forceinline__ f32x4 mfma_tile(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, int K,
+                                           f32x4 acc) {
+    const int lane = threadIdx.x & 63;
+    const uint16_t* pa = A + (lane & 15) * lda + 4 * (lane >> 4);
+    const uint16_t* pb = Bt + (lane & 15) * ldb + 4 * (lane >> 4);
+    for (int k = 0; k < K; k += 16) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(pa + k);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(pb + k);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ float inv_rho_of(const double* scal, uint32_t n) {
+    const double rho = std::sqrt(scal[0] / double(n));
+    return rho > 0.0 ? float(1.0 / rho) : 0.0f;
+}
+__device__ __forceinline__ float rho_of(const double* scal, uint32_t n) {
+    return float(std::sqrt(scal[0] / double(n)));
+}
+
+// s_i = sqrt(inv_diag_i); partial[block] = sum (r_i s_i)^2 (grid stride).
+__global__ __launch_bounds__(kBlock) void scale_kernel(uint32_t n, const float* __restrict__ r,
+                                                      const float* __restrict__ inv_diag, float* __restrict__ s,
+                                                      double* __restrict__ partial) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float si = std::sqrt(inv_diag[i]);
+        s[i] = si;
+        const double v = double(r[i]) * double(si);
+        acc += v * v;
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// h0 = relu(u w_in + b_in); thread per (node, channel), 8 nodes per block.
+__global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t n, const float* __restrict__ r,
+                                                       const float* __restrict__ s, const double* __restrict__ scal,
+                                                       const float* __restrict__ p, uint16_t* __restrict__ h0,
+                                                       uint32_t* __restrict__ mask) {
+    const uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const uint32_t i = uint32_t(t >> 5), c = uint32_t(t & 31);
+    const float inv = inv_rho_of(scal, n);
+    float z = 0.0f;
+    if (i < n) {
+        const float u = r[i] * s[i] * inv;
+        z = u * p[c] + p[kHid + c];
+        h0[size_t(i) * kHid + c] = f2bf(std::max(z, 0.0f));
+    }
+    const uint64_t bits = __ballot(z > 0.0f);
+    if (mask && i < n && c == 0) mask[i] = uint32_t(bits >> (threadIdx.x & 32));
+}
+
+// One message-passing layer: X = [h_i ; sum_j w_ij h_j] (64 x 64, BF16 in
+// LDS), h_out = relu(X W + b) + h_i.
+template <bool kTrain>
+__global__ __launch_bounds__(kBlock) void layer_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                      const uint32_t* __restrict__ cols,
+                                                      const float* __restrict__ vals, const float* __restrict__ s,
+                                                      const uint16_t* __restrict__ hin, uint16_t* __restrict__ hout,
+                                                      const uint16_t* __restrict__ wt_g,
+                                                      const float* __restrict__ bias, uint32_t* __restrict__ mask) {
+    __shared__ __align__(16) uint16_t X[kTile * kLd];
+    __shared__ __align__(16) uint16_t Wt[kHid * kLd];
+    __shared__ float Y[kTile * (kHid + 1)];
+    __shared__ uint8_t Mk[kTile * kHid];
+    const int tid = threadIdx.x;
+    const uint32_t node0 = blockIdx.x * kTile;
+    for (int e = tid; e < kHid * 2 * kHid; e += kBlock) Wt[(e >> 6) * kLd + (e & 63)] = wt_g[e];
+    const int c = tid & 31, sub = tid >> 5;
+    for (int il = sub; il < kTile; il += 8) {
+        const uint32_t i = node0 + il;
+        uint16_t hv = 0;
+        float m = 0.0f;
+        if (i < n) {
+            hv = hin[size_t(i) * kHid + c];
+            const float si = s[i];
+            for (uint32_t k = rowp[i]; k < rowp[i + 1]; ++k) {
+                const uint32_t j = cols[k];
+                if (j == i) continue;
+                const float w = (-vals[k] * si) * s[j];
+                m += w * bf2f(hin[size_t(j) * kHid + c]);
+            }
+        }
+        X[il * kLd + c] = hv;
+        X[il * kLd + kHid + c] = f2bf(m);
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mfma_tile(X + wave * 16 * kLd, kLd, Wt + nt * 16 * kLd, kLd, 2 * kHid, acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = wave * 16 + 4 * (lane >> 4) + j;
+            const int col = nt * 16 + (lane & 15);
+            const float z = acc[j] + bias[col];
+            Y[row * (kHid + 1) + col] = std::max(z, 0.0f) + bf2f(X[row * kLd + col]);
+            if (kTrain) Mk[row * kHid + col] = z > 0.0f ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    for (int il = sub; il < kTile; il += 8) {
+        const uint32_t i = node0 + il;
+        if (i < n) hout[size_t(i) * kHid + c] = f2bf(Y[il * (kHid + 1) + c]);
+    }
+    if (kTrain && tid < kTile && node0 + tid < n) {
+        uint32_t bits = 0;
+        for (int cc = 0; cc < kHid; ++cc) bits |= uint32_t(Mk[tid * kHid + cc]) << cc;
+        mask[node0 + tid] = bits;
+    }
+}
+
+// e_i = rho s_i (h_i . w_out + b_out); thread per node.
+__global__ __launch_bounds__(kBlock) void decode_kernel(uint32_t n, const uint16_t* __restrict__ h,
+                                                       const float* __restrict__ s, const double* __restrict__ scal,
+                                                       const float* __restrict__ p, float* __restrict__ e) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float* w = p + kOffWout;
+    float y = 0.0f;
+    for (int c = 0; c < kHid; ++c) y += bf2f(h[size_t(i) * kHid + c]) * w[c];
+    y += w[kHid];
+    e[i] = rho_of(scal, n) * s[i] * y;
+}
+
+// Sum of acc over the 8 node slots of a 64-node tile, per channel, into
+// out[c] (thread tid < 32 holds channel tid's total on return).
+__device__ __forceinline__ float tile_channel_sum(float acc, float* red /* 8 x 32 */) {
+    const int tid = threadIdx.x;
+    red[(tid >> 5) * kHid + (tid & 31)] = acc;
+    __syncthreads();
+    float t = 0.0f;
+    if (tid < kHid)
+        for (int k = 0; k < 8; ++k) t += red[k * kHid + tid];
+    __syncthreads();
+    return t;
+}
+
+// Decoder backward: gy = ge rho s; gh = gy w_out; partials of w_out, b_out.
+__global__ __launch_bounds__(kBlock) void decode_bwd_kernel(uint32_t n, const float* __restrict__ ge,
+                                                           const float* __restrict__ s,
+                                                           const double* __restrict__ scal,
+                                                           const uint16_t* __restrict__ h, const float* __restrict__ p,
+                                                           float* __restrict__ gh, float* __restrict__ gpart) {
+    __shared__ float red[8 * kHid];
+    const int tid = threadIdx.x, c = tid & 31, sub = tid >> 5;
+    const uint32_t node0 = blockIdx.x * kTile;
+    const float rho = rho_of(scal, n);
+    const float wc = p[kOffWout + c];
+    float aw = 0.0f, ab = 0.0f;
+    for (int il = sub; il < kTile; il += 8) {
+        const uint32_t i = node0 + il;
+        if (i >= n) continue;
+        const float gy = ge[i] * rho * s[i];
+        gh[size_t(i) * kHid + c] = gy * wc;
+        aw += gy * bf2f(h[size_t(i) * kHid + c]);
+        ab += gy;
+    }
+    const float tw = tile_channel_sum(aw, red);
+    const float tb = tile_channel_sum(ab, red);
+    float* out = gpart + size_t(blockIdx.x) * kMpnnParams;
+    if (tid < kHid) out[kOffWout + tid] = tw;
+    if (tid == 0) out[kOffWout + kHid] = tb;  // every channel slot saw the same gy
+}
+
+// Layer backward. In: gh = dL/dh_out, the layer's relu mask, h_prev.
+// Out: gh2 = gh + (dL/dX)[:, :32] (residual + self path), gm = (dL/dX)[:, 32:],
+// per-tile partials of dL/dW (X^T gz) and dL/db (sum gz).
+__global__ __launch_bounds__(kBlock) void layer_bwd_kernel(
+    uint32_t n, const uint32_t* __restrict__ rowp, const uint32_t* __restrict__ cols, const float* __restrict__ vals,
+    const float* __restrict__ s, const uint16_t* __restrict__ hprev, const uint32_t* __restrict__ mask,
+    const uint16_t* __restrict__ w_g, const float* __restrict__ gh, float* __restrict__ gh2, float* __restrict__ gm,
+    float* __restrict__ gpart, uint32_t layer_off) {
+    __shared__ __align__(16) uint16_t XT[2 * kHid * kLd];   // [f][node]
+    __shared__ __align__(16) uint16_t GZ[kTile * kLd32];    // [node][c]
+    __shared__ __align__(16) uint16_t GZT[kHid * kLd];      // [c][node]
+    __shared__ __align__(16) uint16_t W[2 * kHid * kLd32];  // [f][c]
+    __shared__ float G[kTile * (2 * kHid + 1)];             // dL/dX [node][f]
+    __shared__ float red[8 * kHid];
+    const int tid = threadIdx.x;
+    const uint32_t node0 = blockIdx.x * kTile;
+    for (int e = tid; e < 2 * kHid * kHid; e += kBlock) W[(e >> 5) * kLd32 + (e & 31)] = w_g[e];
+    const int c = tid & 31, sub = tid >> 5;
+    float ghv[8];
+    float ab = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int il = sub + 8 * t;
+        const uint32_t i = node0 + il;
+        uint16_t hv = 0;
+        float m = 0.0f, g = 0.0f, gz = 0.0f;
+        if (i < n) {
+            hv = hprev[size_t(i) * kHid + c];
+            const float si = s[i];
+            for (uint32_t k = rowp[i]; k < rowp[i + 1]; ++k) {
+                const uint32_t j = cols[k];
+                if (j == i) continue;
+                const float w = (-vals[k] * si) * s[j];
+                m += w * bf2f(hprev[size_t(j) * kHid + c]);
+            }
+            g = gh[size_t(i) * kHid + c];
+            gz = ((mask[i] >> c) & 1u) ? g : 0.0f;
+        }
+        ghv[t] = g;
+        ab += gz;
+        XT[c * kLd + il] = hv;
+        XT[(kHid + c) * kLd + il] = f2bf(m);
+        const uint16_t gzb = f2bf(gz);
+        GZ[il * kLd32 + c] = gzb;
+        GZT[c * kLd + il] = gzb;
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    // dL/dX = GZ (64 x 32) * W^T (32 x 64): Bt[f][c] = W[f][c].
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mfma_tile(GZ + wave * 16 * kLd32, kLd32, W + nt * 16 * kLd32, kLd32, kHid, acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            G[(wave * 16 + 4 * (lane >> 4) + j) * (2 * kHid + 1) + nt * 16 + (lane & 15)] = acc[j];
+    }
+    // dL/dW = X^T (64 x 64 nodes) * GZ (64 x 32): A = XT rows f, Bt = GZT rows c.
+    float* out = gpart + size_t(blockIdx.x) * kMpnnParams + layer_off;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mfma_tile(XT + wave * 16 * kLd, kLd, GZT + nt * 16 * kLd, kLd, kTile, acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[(wave * 16 + 4 * (lane >> 4) + j) * kHid + nt * 16 + (lane & 15)] = acc[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int il = sub + 8 * t;
+        const uint32_t i = node0 + il;
+        if (i >= n) continue;
+        gh2[size_t(i) * kHid + c] = ghv[t] + G[il * (2 * kHid + 1) + c];
+        gm[size_t(i) * kHid + c] = G[il * (2 * kHid + 1) + kHid + c];
+    }
+    const float tb = tile_channel_sum(ab, red);
+    if (tid < kHid) out[2 * kHid * kHid + tid] = tb;
+}
+
+// gh_out = gh2 + sum_j w_ij gm_j (the mixing is symmetric for symmetric A).
+__global__ __launch_bounds__(kBlock) void mix_t_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                      const uint32_t* __restrict__ cols,
+                                                      const float* __restrict__ vals, const float* __restrict__ s,
+                                                      const float* __restrict__ gm, const float* __restrict__ gh2,
+                                                      float* __restrict__ gh_out) {
+    const uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const uint32_t i = uint32_t(t >> 5), c = uint32_t(t & 31);
+    if (i >= n) return;
+    const float si = s[i];
+    float acc = 0.0f;
+    for (uint32_t k = rowp[i]; k < rowp[i + 1]; ++k) {
+        const uint32_t j = cols[k];
+        if (j == i) continue;
+        const float w = (-vals[k] * si) * s[j];
+        acc += w * gm[size_t(j) * kHid + c];
+    }
+    gh_out[size_t(i) * kHid + c] = gh2[size_t(i) * kHid + c] + acc;
+}
+
+// Encoder backward: partials of w_in (sum u gz) and b_in (sum gz).
+__global__ __launch_bounds__(kBlock) void encode_bwd_kernel(uint32_t n, const float* __restrict__ r,
+                                                           const float* __restrict__ s,
+                                                           const double* __restrict__ scal,
+                                                           const float* __restrict__ gh,
+                                                           const uint32_t* __restrict__ mask,
+                                                           float* __restrict__ gpart) {
+    __shared__ float red[8 * kHid];
+    const int tid = threadIdx.x, c = tid & 31, sub = tid >> 5;
+    const uint32_t node0 = blockIdx.x * kTile;
+    const float inv = inv_rho_of(scal, n);
+    float aw = 0.0f, ab = 0.0f;
+    for (int il = sub; il < kTile; il += 8) {
+        const uint32_t i = node0 + il;
+        if (i >= n) continue;
+        const float g = ((mask[i] >> c) & 1u) ? gh[size_t(i) * kHid + c] : 0.0f;
+        aw += r[i] * s[i] * inv * g;
+        ab += g;
+    }
+    const float tw = tile_channel_sum(aw, red);
+    const float tb = tile_channel_sum(ab, red);
+    float* out = gpart + size_t(blockIdx.x) * kMpnnParams;
+    if (tid < kHid) {
+        out[tid] = tw;
+        out[kHid + tid] = tb;
+    }
+}
+
+// grad[p] = sum over tiles (in tile order) of gpart[tile][p].
+__global__ __launch_bounds__(kBlock) void grad_reduce_kernel(uint32_t tiles, const float* __restrict__ gpart,
+                                                            float* __restrict__ grad) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= uint32_t(kMpnnParams)) return;
+    float acc = 0.0f;
+    for (uint32_t b = 0; b < tiles; ++b) acc += gpart[size_t(b) * kMpnnParams + p];
+    grad[p] = acc;
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) with bias corrections c1 = 1/(1-b1^t),
+// c2 = 1/(1-b2^t).
+__global__ __launch_bounds__(kBlock) void adam_kernel(uint32_t count, float* __restrict__ p, float* __restrict__ m,
+                                                     float* __restrict__ v, const float* __restrict__ g, float lr,
+                                                     float b1, float b2, float eps, float c1, float c2) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (std::sqrt(vi * c2) + eps);
+}
+
+// BF16 copies of the layer weights: w[l][f][c] and wt[l][c][f].
+__global__ __launch_bounds__(kBlock) void weights_bf16_kernel(const float* __restrict__ p, uint16_t* __restrict__ w,
+                                                             uint16_t* __restrict__ wt) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= uint32_t(kMpnnLayers * 2 * kHid * kHid)) return;
+    const uint32_t l = t / (2 * kHid * kHid), e = t % (2 * kHid * kHid);
+    const uint32_t f = e / kHid, c = e % kHid;
+    const uint16_t b = f2bf(p[2 * kHid + l * kLayerStride + e]);
+    w[l * 2 * kHid * kHid + e] = b;
+    wt[l * 2 * kHid * kHid + c * 2 * kHid + f] = b;
+}
+
+// out = x + a * y
+__global__ __launch_bounds__(kBlock) void axpy_kernel(uint32_t n, const float* __restrict__ x, float a,
+                                                     const float* __restrict__ y, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = x[i] + a * y[i];
+}
+
+// ge = -Aw / (e.r); loss = 1/2 v.Av / (e.r).
+__global__ __launch_bounds__(kBlock) void loss_grad_kernel(uint32_t n, const float* __restrict__ aw,
+                                                          const double* __restrict__ scal, float* __restrict__ ge) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) ge[i] = float(-double(aw[i]) / scal[1]);
+}
+__global__ void loss_value_kernel(const double* __restrict__ scal, double* __restrict__ loss) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *loss = 0.5 * scal[2] / scal[1];
+}
+
+inline uint32_t grid_of(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
+inline uint32_t tiles_of(uint32_t n) { return (n + kTile - 1) / kTile; }
+inline uint32_t red_grid(uint32_t n) { return std::min<uint32_t>(grid_of(n), kMaxPartBlocks); }
+
+}  // namespace
+
+Mpnn::Mpnn(int device) : device_(device) {
+    MPNN_HIP(hipSetDevice(device_));
+    p_ = mpnn_alloc<float>(kMpnnParams);
+    m_ = mpnn_alloc<float>(kMpnnParams);
+    v_ = mpnn_alloc<float>(kMpnnParams);
+    grad_ = mpnn_alloc<float>(kMpnnParams);
+    w_ = mpnn_alloc<uint16_t>(kMpnnLayers * 2 * kHid * kHid);
+    wt_ = mpnn_alloc<uint16_t>(kMpnnLayers * 2 * kHid * kHid);
+    scal_ = mpnn_alloc<double>(8);
+    part_ = mpnn_alloc<double>(kMaxPartBlocks);
+    MPNN_HIP(hipMemset(p_, 0, kMpnnParams * sizeof(float)));
+    MPNN_HIP(hipMemset(m_, 0, kMpnnParams * sizeof(float)));
+    MPNN_HIP(hipMemset(v_, 0, kMpnnParams * sizeof(float)));
+    MPNN_HIP(hipMemset(w_, 0, kMpnnLayers * 2 * kHid * kHid * sizeof(uint16_t)));
+    MPNN_HIP(hipMemset(wt_, 0, kMpnnLayers * 2 * kHid * kHid * sizeof(uint16_t)));
+}
+
+Mpnn::~Mpnn() {
+    (void)hipSetDevice(device_);
+    (void)hipDeviceSynchronize();
+    mpnn_free(p_);
+    mpnn_free(m_);
+    mpnn_free(v_);
+    mpnn_free(grad_);
+    mpnn_free(w_);
+    mpnn_free(wt_);
+    mpnn_free(scal_);
+    mpnn_free(part_);
+    mpnn_free(s_);
+    for (auto& h : h_) mpnn_free(h);
+    for (auto& mk : mask_) mpnn_free(mk);
+    mpnn_free(gh_);
+    mpnn_free(gh2_);
+    mpnn_free(gm_);
+    mpnn_free(gpart_);
+    for (auto& t : t_) mpnn_free(t);
+    mpnn_free(pairs_r);
+    mpnn_free(pairs_e);
+}
+
+void Mpnn::ensure(uint32_t n) {
+    if (n <= cap_) return;
+    MPNN_HIP(hipDeviceSynchronize());
+    mpnn_free(s_);
+    for (auto& h : h_) mpnn_free(h);
+    for (auto& mk : mask_) mpnn_free(mk);
+    mpnn_free(gh_);
+    mpnn_free(gh2_);
+    mpnn_free(gm_);
+    mpnn_free(gpart_);
+    for (auto& t : t_) mpnn_free(t);
+    gpart_cap_ = 0;
+    s_ = mpnn_alloc<float>(n);
+    for (auto& h : h_) h = mpnn_alloc<uint16_t>(uint64_t(n) * kHid);
+    for (auto& t : t_) t = mpnn_alloc<float>(n);
+    cap_ = n;
+}
+
+void Mpnn::reserve_pairs(uint32_t n, uint32_t count) {
+    if (pairs_n != n || count > pairs_cap) {
+        MPNN_HIP(hipDeviceSynchronize());
+        mpnn_free(pairs_r);
+        mpnn_free(pairs_e);
+        pairs_r = mpnn_alloc<float>(uint64_t(n) * count);
+        pairs_e = mpnn_alloc<float>(uint64_t(n) * count);
+        pairs_cap = count;
+    }
+    pairs_n = n;
+    pairs_count = 0;
+}
+
+void Mpnn::refresh_bf16(hipStream_t s) {
+    weights_bf16_kernel<<<grid_of(kMpnnLayers * 2 * kHid * kHid), kBlock, 0, s>>>(p_, w_, wt_);
+}
+
+void Mpnn::set_params(const float* host) {
+    MPNN_HIP(hipMemcpy(p_, host, kMpnnParams * sizeof(float), hipMemcpyHostToDevice));
+    MPNN_HIP(hipMemset(m_, 0, kMpnnParams * sizeof(float)));
+    MPNN_HIP(hipMemset(v_, 0, kMpnnParams * sizeof(float)));
+    step_ = 0;
+    refresh_bf16(nullptr);
+    MPNN_HIP(hipDeviceSynchronize());
+}
+
+void Mpnn::get_params(float* host) const {
+    MPNN_HIP(hipDeviceSynchronize());
+    MPNN_HIP(hipMemcpy(host, p_, kMpnnParams * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+void Mpnn::forward(DeviceAmg& amg, const float* r, float* e, bool keep) {
+    const Level0View g = amg.level0();
+    const uint32_t n = g.n;
+    ensure(n);
+    if (keep) {
+        for (auto& mk : mask_)
+            if (!mk) mk = mpnn_alloc<uint32_t>(cap_);
+    }
+    hipStream_t st = amg.stream();
+    if (keep) last_r_ = r;
+    const uint32_t rg = red_grid(n);
+    scale_kernel<<<rg, kBlock, 0, st>>>(n, r, g.inv_diag, s_, part_);
+    reduce_partials_kernel<<<1, 1024, 0, st>>>(part_, rg, kMaxPartBlocks, 1, scal_);
+    const uint32_t gn = uint32_t((uint64_t(n) * kHid + kBlock - 1) / kBlock);
+    encode_kernel<<<gn, kBlock, 0, st>>>(n, r, s_, scal_, p_, h_[0], keep ? mask_[0] : nullptr);
+    const uint32_t tiles = tiles_of(n);
+    for (int l = 0; l < kMpnnLayers; ++l) {
+        const float* bias = p_ + 2 * kHid + l * kLayerStride + 2 * kHid * kHid;
+        const uint16_t* wt = wt_ + l * 2 * kHid * kHid;
+        if (keep)
+            layer_kernel<true><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], h_[l + 1], wt, bias,
+                                                         mask_[l + 1]);
+        else
+            layer_kernel<false><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], h_[l + 1], wt, bias,
+                                                          nullptr);
+    }
+    decode_kernel<<<grid_of(n), kBlock, 0, st>>>(n, h_[kMpnnLayers], s_, scal_, p_, e);
+}
+
+void Mpnn::backward(DeviceAmg& amg, const float* ge, float* grad_dev) {
+    const Level0View g = amg.level0();
+    const uint32_t n = g.n;
+    hipStream_t st = amg.stream();
+    const uint32_t tiles = tiles_of(n);
+    if (!gh_) {
+        gh_ = mpnn_alloc<float>(uint64_t(cap_) * kHid);
+        gh2_ = mpnn_alloc<float>(uint64_t(cap_) * kHid);
+        gm_ = mpnn_alloc<float>(uint64_t(cap_) * kHid);
+    }
+    const uint64_t need = uint64_t(tiles_of(cap_)) * kMpnnParams;
+    if (gpart_cap_ < need) {
+        MPNN_HIP(hipStreamSynchronize(st));
+        mpnn_free(gpart_);
+        gpart_ = mpnn_alloc<float>(need);
+        gpart_cap_ = need;
+    }
+    decode_bwd_kernel<<<tiles, kBlock, 0, st>>>(n, ge, s_, scal_, h_[kMpnnLayers], p_, gh_, gpart_);
+    const uint32_t gn = uint32_t((uint64_t(n) * kHid + kBlock - 1) / kBlock);
+    for (int l = kMpnnLayers - 1; l >= 0; --l) {
+        const uint32_t off = 2 * kHid + l * kLayerStride;
+        layer_bwd_kernel<<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], mask_[l + 1],
+                                                   w_ + l * 2 * kHid * kHid, gh_, gh2_, gm_, gpart_, off);
+        mix_t_kernel<<<gn, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, gm_, gh2_, gh_);
+    }
+    encode_bwd_kernel<<<tiles, kBlock, 0, st>>>(n, last_r_, s_, scal_, gh_, mask_[0], gpart_);
+    grad_reduce_kernel<<<grid_of(kMpnnParams), kBlock, 0, st>>>(tiles, gpart_, grad_dev);
+}
+
+void Mpnn::adam_step(const float* grad_dev, float lr, hipStream_t s) {
+    ++step_;
+    const double b1 = 0.9, b2 = 0.999;
+    const float c1 = float(1.0 / (1.0 - std::pow(b1, double(step_))));
+    const float c2 = float(1.0 / (1.0 - std::pow(b2, double(step_))));
+    adam_kernel<<<grid_of(kMpnnParams), kBlock, 0, s>>>(kMpnnParams, p_, m_, v_, grad_dev, lr, float(b1), float(b2),
+                                                        1e-8f, c1, c2);
+    refresh_bf16(s);
+}
+
+bool Mpnn::apply(DeviceAmg& amg, const float* r, float* z, std::string& err) {
+    const uint32_t n = amg.level0().n;
+    ensure(n);
+    float* e = t_[0];
+    float* res = t_[1];
+    float* zv = t_[2];
+    forward(amg, r, e, false);
+    amg.residual32(r, e, res);
+    if (!amg.vcycle32(res, zv)) {
+        err = "learned stage: V-cycle failed";
+        return false;
+    }
+    axpy_kernel<<<grid_of(n), kBlock, 0, amg.stream()>>>(n, e, 1.0f, zv, z);
+    return true;
+}
+
+bool Mpnn::train_step(DeviceAmg& amg, const float* r, const float* e, float lr, double* loss_dev,
+                      std::string& err) {
+    const uint32_t n = amg.level0().n;
+    ensure(n);
+    hipStream_t st = amg.stream();
+    float* enn = t_[0];
+    float* u = t_[1];
+    float* a = t_[2];
+    float* b = t_[3];
+    float* v = t_[4];
+    forward(amg, r, enn, true);
+    const uint32_t g = grid_of(n), rg = red_grid(n);
+    axpy_kernel<<<g, kBlock, 0, st>>>(n, e, -1.0f, enn, u);  // u = e - N(r)
+    amg.spmv32(u, a);                                        // a = A u
+    if (!amg.vcycle32(a, b)) {                               // b = V A u
+        err = "training: V-cycle failed";
+        return false;
+    }
+    axpy_kernel<<<g, kBlock, 0, st>>>(n, u, -1.0f, b, v);    // v = E u
+    amg.spmv32(v, a);                                        // a = A v
+    dot_partial_kernel<float><<<rg, kBlock, 0, st>>>(n, v, a, part_);
+    reduce_partials_kernel<<<1, 1024, 0, st>>>(part_, rg, kMaxPartBlocks, 1, scal_ + 2);
+    dot_partial_kernel<float><<<rg, kBlock, 0, st>>>(n, e, r, part_);
+    reduce_partials_kernel<<<1, 1024, 0, st>>>(part_, rg, kMaxPartBlocks, 1, scal_ + 1);
+    if (!amg.vcycle32(a, b)) {                               // b = V A v
+        err = "training: V-cycle failed";
+        return false;
+    }
+    axpy_kernel<<<g, kBlock, 0, st>>>(n, v, -1.0f, b, u);    // u = E v
+    amg.spmv32(u, a);                                        // a = A E E u
+    loss_grad_kernel<<<g, kBlock, 0, st>>>(n, a, scal_, b);  // b = dL/dN(r)
+    if (loss_dev) loss_value_kernel<<<1, 1, 0, st>>>(scal_, loss_dev);
+    backward(amg, b, grad_);
+    adam_step(grad_, lr, st);
+    return true;
+}
+
+// ------------------------------------------------- host-vector entries --
+
+namespace {
+struct Scoped {
+    void* p = nullptr;
+    explicit Scoped(size_t bytes) { MPNN_HIP(hipMalloc(&p, bytes ? bytes : 1)); }
+    ~Scoped() { (void)hipFree(p); }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+}  // namespace
+
+void mpnn_forward_host(Mpnn& m, DeviceAmg& amg, const double* r, double* e) {
+    const uint32_t n = amg.level0().n;
+    hipStream_t st = amg.stream();
+    Scoped rd(size_t(n) * 8), rs(size_t(n) * 4), es(size_t(n) * 4);
+    MPNN_HIP(hipMemcpy(rd.p, r, size_t(n) * 8, hipMemcpyHostToDevice));
+    scatter_perm_kernel<double, float><<<grid_of(n), kBlock, 0, st>>>(n, amg.device_perm0(), rd.as<double>(),
+                                                                     rs.as<float>());
+    m.forward(amg, rs.as<float>(), es.as<float>(), false);
+    gather_perm_kernel<float, double><<<grid_of(n), kBlock, 0, st>>>(n, amg.device_perm0(), es.as<float>(),
+                                                                    rd.as<double>());
+    MPNN_HIP(hipStreamSynchronize(st));
+    MPNN_HIP(hipMemcpy(e, rd.p, size_t(n) * 8, hipMemcpyDeviceToHost));
+}
+
+void mpnn_vjp_host(Mpnn& m, DeviceAmg& amg, const double* r, const double* ge, float* grad) {
+    const uint32_t n = amg.level0().n;
+    hipStream_t st = amg.stream();
+    Scoped rd(size_t(n) * 8), rs(size_t(n) * 4), gs(size_t(n) * 4), es(size_t(n) * 4), gd(kMpnnParams * 4);
+    MPNN_HIP(hipMemcpy(rd.p, r, size_t(n) * 8, hipMemcpyHostToDevice));
+    scatter_perm_kernel<double, float><<<grid_of(n), kBlock, 0, st>>>(n, amg.device_perm0(), rd.as<double>(),
+                                                                     rs.as<float>());
+    MPNN_HIP(hipStreamSynchronize(st));
+    MPNN_HIP(hipMemcpy(rd.p, ge, size_t(n) * 8, hipMemcpyHostToDevice));
+    scatter_perm_kernel<double, float><<<grid_of(n), kBlock, 0, st>>>(n, amg.device_perm0(), rd.as<double>(),
+                                                                     gs.as<float>());
+    m.forward(amg, rs.as<float>(), es.as<float>(), true);
+    m.backward(amg, gs.as<float>(), gd.as<float>());
+    MPNN_HIP(hipStreamSynchronize(st));
+    MPNN_HIP(hipMemcpy(grad, gd.p, kMpnnParams * 4, hipMemcpyDeviceToHost));
+}
+
+void mpnn_adam_host(Mpnn& m, const float* grad, float lr) {
+    Scoped gd(kMpnnParams * 4);
+    MPNN_HIP(hipMemcpy(gd.p, grad, kMpnnParams * 4, hipMemcpyHostToDevice));
+    m.adam_step(gd.as<float>(), lr, nullptr);
+    MPNN_HIP(hipDeviceSynchronize());
+}
+
+bool mpnn_collect(Mpnn& m, DeviceAmg& amg, const double* b, const std::vector<uint64_t>& iterations, double tol,
+                  uint32_t& got, SolveStats& st, std::string& err) {
+    const uint32_t n = amg.level0().n;
+    m.reserve_pairs(n, uint32_t(iterations.size()));
+    Scoped bd(size_t(n) * 8);
+    MPNN_HIP(hipMemcpy(bd.p, b, size_t(n) * 8, hipMemcpyHostToDevice));
+    const bool ok = amg.collect_pairs(bd.as<double>(), iterations, tol, m.pairs_r, m.pairs_e, got, st, err);
+    m.pairs_count = ok ? got : 0;
+    return ok;
+}
+
+bool mpnn_train(Mpnn& m, DeviceAmg& amg, uint32_t steps, float lr, double* losses, std::string& err) {
+    const uint32_t n = amg.level0().n;
+    if (m.pairs_count == 0 || m.pairs_n != n) {
+        err = "training needs pairs collected on this hierarchy";
+        return false;
+    }
+    Scoped ld(size_t(steps) * 8);
+    for (uint32_t k = 0; k < steps; ++k) {
+        const size_t off = size_t(k % m.pairs_count) * n;
+        if (!m.train_step(amg, m.pairs_r + off, m.pairs_e + off, lr, ld.as<double>() + k, err)) return false;
+    }
+    MPNN_HIP(hipStreamSynchronize(amg.stream()));
+    if (losses) MPNN_HIP(hipMemcpy(losses, ld.p, size_t(steps) * 8, hipMemcpyDeviceToHost));
+    return true;
+}
+
+}  // namespace cfdxhip

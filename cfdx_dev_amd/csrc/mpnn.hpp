@@ -1,0 +1,114 @@
+// Learned preconditioner stage: a 4-layer message-passing network (hidden
+// width 32) on the graph of the finest-level operator, run ahead of one FP32
+// V-cycle inside mixed FGMRES:
+//
+//     e = N(r),   z = e + V(r - A e)
+//
+// Network (per node i, s_i = 1/sqrt(a_ii), w_ij = -a_ij s_i s_j):
+//     u_i  = r_i s_i / rho,  rho = sqrt(mean (r s)^2)
+//     h0_i = relu(u_i w_in + b_in)
+//     layer l: m_i = sum_j w_ij h_j;  h_i <- relu(W_l^T [h_i; m_i] + b_l) + h_i
+//     e_i  = rho s_i (h_i . w_out + b_out)
+// Hidden states are stored in BF16; the dense 64 -> 32 layer products run on
+// MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; parameters, Adam
+// moments and all reductions are FP32/FP64.
+//
+// Training (on the GPU, one (r, e) pair per step) minimises the energy norm of
+// the error left after the whole stage,
+//     L = 1/2 |E (e - N(r))|_A^2 / (e^T r),   E = I - V A,
+// whose gradient with respect to N(r) is -A E E (e - N(r)) / (e^T r) for a
+// symmetric V-cycle V (three SpMVs and two V-cycles per step), followed by
+// the network's backward pass and one Adam step (Kingma & Ba 2015).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device_amg.hpp"
+
+namespace cfdxhip {
+
+constexpr int kHid = 32;
+constexpr int kMpnnLayers = 4;
+constexpr int kLayerStride = 2 * kHid * kHid + kHid;          // W_l (64 x 32) + b_l
+constexpr int kOffWout = 2 * kHid + kMpnnLayers * kLayerStride;  // 8384
+constexpr int kMpnnParams = kOffWout + kHid + 1;              // 8417
+
+class Mpnn : public PreconditionerStage {
+public:
+    explicit Mpnn(int device);
+    ~Mpnn() override;
+    Mpnn(const Mpnn&) = delete;
+    Mpnn& operator=(const Mpnn&) = delete;
+
+    int device() const { return device_; }
+    // Parameters (flat layout above); setting them resets the Adam state.
+    void set_params(const float* host);
+    void get_params(float* host) const;
+
+    // e = N(r), FP32 vectors in the solve ordering of amg.
+    void forward(DeviceAmg& amg, const float* r, float* e, bool keep);
+    // grad = d(ge . N(r)) / dparams for the r of the last forward(keep=true).
+    void backward(DeviceAmg& amg, const float* ge, float* grad_dev);
+    void adam_step(const float* grad_dev, float lr, hipStream_t s);
+
+    // PreconditionerStage: z = e + V(r - A e).
+    bool apply(DeviceAmg& amg, const float* r, float* z, std::string& err) override;
+
+    // One training step on a (r, e) pair (device FP32, solve ordering);
+    // returns the loss of this step's forward pass through `loss_dev`.
+    bool train_step(DeviceAmg& amg, const float* r, const float* e, float lr, double* loss_dev, std::string& err);
+
+    // Stored pairs of the last collection (device), n per pair.
+    float* pairs_r = nullptr;
+    float* pairs_e = nullptr;
+    uint32_t pairs_n = 0, pairs_count = 0, pairs_cap = 0;
+    void reserve_pairs(uint32_t n, uint32_t count);
+
+    // Reference per-iteration rate (log10 residual reduction per FGMRES
+    // iteration) of the plain mixed method, and the fallback ratio.
+    double reference_rate = 0.0;
+    double fallback_ratio = 0.5;
+    uint32_t fallback_window = 5;
+
+private:
+    void ensure(uint32_t n);
+    void refresh_bf16(hipStream_t s);
+
+    int device_ = 0;
+    uint32_t cap_ = 0;      // node capacity of the workspace
+    const float* last_r_ = nullptr;  // input of the last forward(keep = true)
+    // parameters and optimizer state (device, FP32)
+    float* p_ = nullptr;
+    float* m_ = nullptr;
+    float* v_ = nullptr;
+    uint32_t step_ = 0;
+    uint16_t* wt_ = nullptr;  // per layer W_l^T as BF16, [32][64]
+    uint16_t* w_ = nullptr;   // per layer W_l as BF16, [64][32]
+    // workspace
+    float* s_ = nullptr;        // 1/sqrt(a_ii)
+    double* scal_ = nullptr;    // device scalars: 0 sum (r s)^2, 1 e.r, 2 v.Av
+    double* part_ = nullptr;    // reduction partials
+    uint16_t* h_[kMpnnLayers + 1] = {};  // BF16 hidden states
+    uint32_t* mask_[kMpnnLayers + 1] = {};
+    float* gh_ = nullptr;       // FP32 n x 32 gradient buffers
+    float* gh2_ = nullptr;
+    float* gm_ = nullptr;
+    float* gpart_ = nullptr;    // per-block parameter-gradient partials
+    uint64_t gpart_cap_ = 0;
+    float* grad_ = nullptr;     // kMpnnParams
+    float* t_[6] = {};          // FP32 n-vectors for the stage and the loss
+};
+
+// Entries for the C ABI: host FP64 vectors in the caller's ordering.
+void mpnn_forward_host(Mpnn& m, DeviceAmg& amg, const double* r, double* e);
+void mpnn_vjp_host(Mpnn& m, DeviceAmg& amg, const double* r, const double* ge, float* grad);
+void mpnn_adam_host(Mpnn& m, const float* grad, float lr);
+bool mpnn_collect(Mpnn& m, DeviceAmg& amg, const double* b, const std::vector<uint64_t>& iterations, double tol,
+                  uint32_t& got, SolveStats& st, std::string& err);
+bool mpnn_train(Mpnn& m, DeviceAmg& amg, uint32_t steps, float lr, double* losses, std::string& err);
+
+}  // namespace cfdxhip

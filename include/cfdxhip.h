@@ -82,7 +82,7 @@ enum { CFDXHIP_PRECISION_FP64 = 0, CFDXHIP_PRECISION_FP32 = 1 };
  * CHEBYSHEV has no counterpart inside the reference's V-cycle: the textbook
  * Jacobi-preconditioned three-term Chebyshev recurrence of chebyshev_degree
  * steps per smoothing on [1.1 lambda / ratio, 1.1 lambda], lambda being a
- * 10-step power-iteration estimate of lambda_max(D^-1 A) per level. (The
+ * 20-step Lanczos estimate of lambda_max(D^-1 A) per level. (The
  * reference's standalone ChebyshevSmoother, chebyshev_smoother.h:11-118, uses
  * a different update and is not on its V-cycle path.) */
 enum { CFDXHIP_SMOOTHER_JACOBI = 0, CFDXHIP_SMOOTHER_CHEBYSHEV = 1 };
@@ -152,10 +152,14 @@ cfdxhip_status cfdxhip_amg_level_matrix(const cfdxhip_amg* h, size_t level, uint
 cfdxhip_status cfdxhip_amg_level_transfer(const cfdxhip_amg* h, size_t level, uint64_t* aggregate,
                                           uint32_t* p_row_offsets, uint32_t* p_columns, double* p_values);
 cfdxhip_status cfdxhip_amg_smoother_omega(const cfdxhip_amg* h, double* omega);
-/* Power-iteration estimate of lambda_max(D^-1 A) on a level, the Chebyshev
+/* Lanczos estimate of lambda_max(D^-1 A) on a level, the Chebyshev
  * smoother's interval anchor. CFDXHIP_ERR_NOT_READY unless a Chebyshev
  * smoother is configured. No reference counterpart. */
 cfdxhip_status cfdxhip_amg_level_lambda_max(const cfdxhip_amg* h, size_t level, double* lambda_max);
+/* Changes the Chebyshev degree and interval ratio of a set-up hierarchy that
+ * uses a Chebyshev smoother (coefficients only, no new setup). No reference
+ * counterpart. */
+cfdxhip_status cfdxhip_amg_set_chebyshev(cfdxhip_amg* h, uint32_t degree, double ratio);
 /* Seconds spent in the last setup: host construction, device upload. */
 cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* host_seconds, double* device_seconds);
 
@@ -252,6 +256,56 @@ cfdxhip_status cfdxhip_fv_diffusion_assemble(cfdxhip_fv* fv, double gamma, const
                                              const double* face_value, const uint8_t* face_is_neumann,
                                              size_t* nnz, uint32_t* row_offsets, uint32_t* columns,
                                              double* values, double* rhs);
+
+/* ------------------------------------------------------------------------
+ * Learned preconditioner stage. No reference counterpart (the reference has
+ * no learned component). A 4-layer message-passing network, hidden width 32,
+ * on the graph of the finest operator of a set-up FP32 (mixed) hierarchy:
+ * features are the symmetrically scaled couplings -a_ij / sqrt(a_ii a_jj)
+ * and the scaled residual r_i / sqrt(a_ii). Inside mixed FGMRES the stage is
+ * z = N(r) + V(r - A N(r)), V one FP32 V-cycle. BF16 MFMA, FP32 accumulation.
+ * The graph must be symmetric (the pressure Laplacian is).
+ * ------------------------------------------------------------------------ */
+typedef struct cfdxhip_mpnn cfdxhip_mpnn;
+
+/* Number of network parameters (8417). Layout: w_in[32], b_in[32]; per layer
+ * l = 0..3 at 64 + 2080 l: W_l[64][32] row-major (rows: [h; m] features),
+ * b_l[32]; then w_out[32], b_out. */
+size_t cfdxhip_mpnn_param_count(void);
+cfdxhip_status cfdxhip_mpnn_create(int32_t device, cfdxhip_mpnn** out);
+void cfdxhip_mpnn_destroy(cfdxhip_mpnn* m);
+const char* cfdxhip_mpnn_last_error(const cfdxhip_mpnn* m);
+/* Sets the FP32 parameters (resets the Adam moments and step count). */
+cfdxhip_status cfdxhip_mpnn_set_params(cfdxhip_mpnn* m, const float* params, size_t count);
+cfdxhip_status cfdxhip_mpnn_get_params(cfdxhip_mpnn* m, float* params, size_t count);
+/* e = N(r) on the graph of h's finest level; host FP64 vectors, caller order. */
+cfdxhip_status cfdxhip_mpnn_forward(cfdxhip_mpnn* m, cfdxhip_amg* h, const double* r, double* e);
+/* grad = d(ge . N(r)) / d params (the network's backward pass). */
+cfdxhip_status cfdxhip_mpnn_vjp(cfdxhip_mpnn* m, cfdxhip_amg* h, const double* r, const double* ge, float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with a host gradient. */
+cfdxhip_status cfdxhip_mpnn_adam_step(cfdxhip_mpnn* m, const float* grad, float lr);
+/* Solver-generated training pairs: FP64 PCG from x = 0 to tol on b (host,
+ * caller order) with h's FP64 V-cycle; at each listed iteration k the pair
+ * (r_k = b - A x_k, e_k = x_final - x_k) is kept on the device. */
+cfdxhip_status cfdxhip_mpnn_collect_pairs(cfdxhip_mpnn* m, cfdxhip_amg* h, const double* b,
+                                          const uint64_t* iterations, uint32_t count, double tol,
+                                          uint32_t* collected, cfdxhip_result* solve);
+/* `steps` Adam steps over the collected pairs (cyclically), minimising
+ * 1/2 |(I - V A)(e - N(r))|_A^2 / (e^T r). losses[steps] (optional) receives
+ * each step's loss. */
+cfdxhip_status cfdxhip_mpnn_train(cfdxhip_mpnn* m, cfdxhip_amg* h, uint32_t steps, float lr, double* losses);
+/* Fallback rule: inside a learned solve the stage is dropped (plain V-cycle,
+ * counted in learned_stage_fallbacks) once the residual reduction per
+ * iteration, log10(|g_{k-window}| / |g_k|) / window, falls below
+ * ratio * reference_rate; reference_rate is the plain method's rate. */
+cfdxhip_status cfdxhip_mpnn_set_fallback(cfdxhip_mpnn* m, double reference_rate, double ratio, uint32_t window);
+/* Mixed FGMRES with the learned stage ahead of the FP32 V-cycle (host or
+ * device vectors). Same contract as cfdxhip_amg_solve_fgmres. */
+cfdxhip_status cfdxhip_amg_solve_fgmres_learned(cfdxhip_amg* h, cfdxhip_mpnn* m, const double* b, double* x,
+                                                int restart, size_t max_iter, double tol, cfdxhip_result* result);
+cfdxhip_status cfdxhip_amg_solve_fgmres_learned_device(cfdxhip_amg* h, cfdxhip_mpnn* m, const double* b,
+                                                       double* x, int restart, size_t max_iter, double tol,
+                                                       cfdxhip_result* result);
 
 /* Version string of the library build. */
 const char* cfdxhip_version(void);

@@ -728,35 +728,77 @@ struct Amg {
     std::string error;
 };
 
-// Power iteration x <- D^-1 A x from the hashed start vector
-// (((i * 2654435761 + 0x9e3779b9) mod 2^32) >> 8) / 2^24 - 1/2, 10 steps,
-// returning x^T A x / x^T D x of the last iterate. Sums run over 8192-row
-// blocks (row order inside a block), then over blocks in order.
-double lambda_estimate(const Level& L, size_t iterations) {
+// Largest eigenvalue of the symmetric tridiagonal (a, b[1..]) by bisection:
+// Sturm count of eigenvalues below x from q_i = a_i - x - b_i^2 / q_{i-1}.
+double tridiag_top(const std::vector<double>& a, const std::vector<double>& b) {
+    const size_t m = a.size();
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    for (size_t i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::abs(b[i]) : 0.0) + (i + 1 < m ? std::abs(b[i + 1]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::abs(hi)); ++it) {
+        const double x = 0.5 * (lo + hi);
+        size_t below = 0;
+        double q = 1.0;
+        for (size_t i = 0; i < m; ++i) {
+            q = a[i] - x - (i > 0 ? b[i] * b[i] / q : 0.0);
+            if (q == 0.0) q = -1e-300;
+            if (q < 0.0) ++below;
+        }
+        if (below >= m) hi = x;
+        else lo = x;
+    }
+    return hi;
+}
+
+// 20 Lanczos steps on D^-1/2 A D^-1/2 from the hashed start vector
+// (((i * 2654435761 + 0x9e3779b9) mod 2^32) >> 8) / 2^24 - 1/2, normalised;
+// the estimate is the largest Ritz value. Sums run over 8192-row blocks (row
+// order inside a block), then over blocks in order.
+double lambda_estimate(const Level& L, size_t steps) {
     const Csr& A = L.A;
     const size_t n = A.n, rows = 8192;
-    std::vector<double> x(n), y(n);
-    for (size_t i = 0; i < n; ++i) {
-        const uint32_t hsh = uint32_t(i) * 2654435761u + 0x9e3779b9u;
-        x[i] = double(hsh >> 8) / 16777216.0 - 0.5;
-    }
-    double lambda = 0.0;
-    for (size_t it = 0; it < iterations; ++it) {
-        matvec(A, x.data(), y.data());
-        double tn = 0.0, td = 0.0;
+    std::vector<double> sq(n), v(n), vp(n, 0.0), w(n);
+    auto bsum = [&](auto&& f) {
+        double t = 0.0;
         for (size_t b0 = 0; b0 < n; b0 += rows) {
-            double sn = 0.0, sd = 0.0;
-            for (size_t i = b0; i < std::min(n, b0 + rows); ++i) {
-                sn += x[i] * y[i];
-                sd += x[i] * x[i] / L.inv_diag[i];
-            }
-            tn += sn;
-            td += sd;
+            double acc = 0.0;
+            for (size_t i = b0; i < std::min(n, b0 + rows); ++i) acc += f(i);
+            t += acc;
         }
-        lambda = tn / td;
-        for (size_t i = 0; i < n; ++i) x[i] = L.inv_diag[i] * y[i];
+        return t;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        sq[i] = std::sqrt(L.inv_diag[i]);
+        const uint32_t hsh = uint32_t(i) * 2654435761u + 0x9e3779b9u;
+        v[i] = double(hsh >> 8) / 16777216.0 - 0.5;
     }
-    return lambda;
+    const double nv = std::sqrt(bsum([&](size_t i) { return v[i] * v[i]; }));
+    for (size_t i = 0; i < n; ++i) v[i] /= nv;
+    std::vector<double> alpha, beta(1, 0.0);
+    steps = std::min(steps, n);
+    for (size_t k = 0; k < steps; ++k) {
+        for (size_t i = 0; i < n; ++i) {
+            double acc = 0.0;
+            for (uint32_t q = A.rowp[i]; q < A.rowp[i + 1]; ++q) acc += A.vals[q] * (sq[A.cols[q]] * v[A.cols[q]]);
+            w[i] = sq[i] * acc;
+        }
+        const double a = bsum([&](size_t i) { return w[i] * v[i]; });
+        alpha.push_back(a);
+        const double bp = beta.back();
+        for (size_t i = 0; i < n; ++i) w[i] = (w[i] - a * v[i]) - bp * vp[i];
+        const double bn = std::sqrt(bsum([&](size_t i) { return w[i] * w[i]; }));
+        if (!(bn > 1e-14 * std::abs(a)) || k + 1 == steps) break;
+        beta.push_back(bn);
+        for (size_t i = 0; i < n; ++i) {
+            vp[i] = v[i];
+            v[i] = w[i] / bn;
+        }
+    }
+    beta.resize(alpha.size());
+    return tridiag_top(alpha, beta);
 }
 
 // Chebyshev three-term recurrence on [hi/ratio, hi], hi = 1.1 lambda
@@ -783,7 +825,7 @@ void cheb_coeffs(double lambda, double ratio, size_t degree, std::vector<double>
 void fill_lambda(Amg& h) {
     h.lambda.clear();
     if (h.smoother == 1)
-        for (const auto& L : h.lv) h.lambda.push_back(lambda_estimate(L, 10));
+        for (const auto& L : h.lv) h.lambda.push_back(lambda_estimate(L, 20));
 }
 
 bool csr_valid(const Csr& A) {

@@ -5,8 +5,8 @@ Pinned without the reference:
     T_k((theta - lam)/delta) / T_k(theta/delta) on the damped interval;
   * a two-level V-cycle with an exact (LU) coarse solve against its closed
     form (I - E) A^-1 r, E = p(D^-1 A) (I - P Ac^-1 P^T A) p(D^-1 A);
-  * the lambda_max estimate as a lower bound within 15 % of the true
-    largest eigenvalue of D^-1 A;
+  * the lambda_max estimate (20 Lanczos steps) as a lower bound within 3 %
+    of the true largest eigenvalue of D^-1 A;
   * the product's host estimate equal to the oracle's, bit for bit.
 """
 import numpy as np
@@ -92,7 +92,7 @@ def test_lambda_estimate_bounds(oracle):
         true = np.linalg.eigvalsh(A / np.sqrt(np.outer(d, d)))[-1]
         est = a.lambda_max(l)
         assert est <= true * (1 + 1e-12), (l, est, true)
-        assert est >= 0.85 * true, (l, est, true)
+        assert est >= 0.97 * true, (l, est, true)
 
 
 def test_host_lambda_matches_oracle(oracle):

@@ -1,0 +1,104 @@
+"""Learned stage on the GPU: HIP forward/backward/Adam against the CPU oracle
+(oracle/mpnn_oracle.py), training on solver-generated pairs, the fallback
+rule, and the ablation (mixed FGMRES with vs without the trained network).
+
+Tolerances: hidden states are BF16 (8-bit mantissa) and the dense products
+run on BF16 MFMA with FP32 accumulation, so the forward output is compared to
+the FP64 oracle at 3e-2 relative (2-norm) and the parameter gradient by
+cosine similarity >= 0.995 and norm ratio within 5 %.
+"""
+import numpy as np
+import pytest
+
+import cfdx_dev_amd as cx
+import mpnn_oracle as M
+from systems import box_mesh, pressure_system
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(oracle, dims, seed, sigma=1.0):
+    return pressure_system(oracle, box_mesh(*dims, jitter=0.15, seed=seed), sigma=sigma, seed=seed)
+
+
+def _amg(s, **kw):
+    h = cx.NativeAMGPreconditioner(precision="fp32", smoother_fp32="chebyshev", **kw)
+    assert h.setup(s["n"], s["rowp"], s["cols"], s["vals"]), h.last_error()
+    return h
+
+
+def _params(seed):
+    p = M.init_params(seed, out_scale=1.0).astype(np.float64)
+    p += 0.05 * np.random.default_rng(seed + 100).standard_normal(M.NPARAMS)
+    return p.astype(np.float32)
+
+
+@pytest.mark.parametrize("dims", [(10, 9, 8), (17, 13, 11)])
+def test_forward_matches_oracle(oracle, dims):
+    s = _system(oracle, dims, 11)
+    h = _amg(s)
+    p = _params(1)
+    net = cx.LearnedStage(params=p)
+    g = M.Graph(s["rowp"], s["cols"], s["vals"])
+    rng = np.random.default_rng(2)
+    for _ in range(2):
+        r = rng.standard_normal(s["n"])
+        eo = M.forward(g, p, r)
+        eg = net.forward(h, r)
+        rel = np.linalg.norm(eg - eo) / np.linalg.norm(eo)
+        assert rel <= 3e-2, rel
+    assert np.all(net.forward(h, np.zeros(s["n"])) == 0.0)
+
+
+def test_vjp_matches_oracle(oracle):
+    s = _system(oracle, (12, 10, 9), 12)
+    h = _amg(s)
+    p = _params(3)
+    net = cx.LearnedStage(params=p)
+    g = M.Graph(s["rowp"], s["cols"], s["vals"])
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal(s["n"])
+    ge = rng.standard_normal(s["n"])
+    go = M.vjp(g, p, r, ge)
+    gg = net.vjp(h, r, ge).astype(np.float64)
+    blocks = [(0, 64)] + [(64 + 2080 * l, 64 + 2080 * (l + 1)) for l in range(4)] + [(M.OFF_WOUT, M.NPARAMS)]
+    for a, b in blocks:
+        x, y = gg[a:b], go[a:b]
+        cos = x @ y / (np.linalg.norm(x) * np.linalg.norm(y))
+        ratio = np.linalg.norm(x) / np.linalg.norm(y)
+        assert cos >= 0.995 and abs(ratio - 1) <= 0.05, (a, b, cos, ratio)
+
+
+def test_adam_step_closed_form():
+    p0 = _params(5)
+    net = cx.LearnedStage(params=p0)
+    g = np.random.default_rng(6).standard_normal(M.NPARAMS).astype(np.float32)
+    net.adam_step(g, 1e-3)
+    p1 = net.get_params().astype(np.float64)
+    expect = p0.astype(np.float64) - 1e-3 * g / (np.abs(g) + 1e-8)
+    np.testing.assert_allclose(p1, expect, rtol=0, atol=2e-6)
+    # matches the oracle's Adam over two steps
+    net.adam_step(g, 1e-3)
+    q, m, v = M.adam(p0.astype(np.float64), np.zeros(M.NPARAMS), np.zeros(M.NPARAMS), g.astype(np.float64), 1, 1e-3)
+    q, m, v = M.adam(q, m, v, g.astype(np.float64), 2, 1e-3)
+    np.testing.assert_allclose(net.get_params(), q, rtol=0, atol=4e-6)
+
+
+def test_training_reduces_loss_and_fallback(oracle):
+    s = _system(oracle, (24, 22, 20), 13, sigma=1.5)
+    h = _amg(s)
+    net = cx.LearnedStage(params=M.init_params(7))
+    got, res = net.collect_pairs(h, s["b"], [0, 1, 2, 3, 4, 6, 8, 10], tol=1e-10)
+    assert res.status == 0 and got == 8
+    losses = net.train(h, 240, 2e-3)
+    assert np.all(np.isfinite(losses))
+    first, last = losses[:24].mean(), losses[-24:].mean()
+    assert last < 0.8 * first, (first, last)
+    # Fallback: an unreachable reference rate drops the stage after the window.
+    net.set_fallback(reference_rate=100.0, ratio=0.5, window=3)
+    x = np.zeros(s["n"])
+    r = h.solve_fgmres_learned(s["b"], x, net, 30, 500, 1e-10)
+    assert r.status == 0 and r.extra["learned_stage_fallbacks"] == 1
+    assert r.extra["learned_stage_applications"] == 3
+    y = oracle.spmv(s["rowp"], s["cols"], s["vals"], x)
+    assert np.linalg.norm(s["b"] - y) / np.linalg.norm(s["b"]) <= 1.01e-10
