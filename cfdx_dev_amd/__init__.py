@@ -406,6 +406,24 @@ class LearnedStage:
         if st != 0:
             raise RuntimeError(f"status {st}: {self.last_error()}")
 
+    @staticmethod
+    def initial_params(seed: int = 0, out_scale: float = 0.1) -> np.ndarray:
+        """Seeded initialisation (layout of cfdxhip_mpnn_param_count): w_in ~
+        N(0, 1), layer weights ~ U(+-sqrt(6/96)/2), w_out ~ out_scale N(0, 1/32),
+        biases 0. A small w_out starts the stage close to the plain V-cycle."""
+        H, L = 32, 4
+        stride = 2 * H * H + H
+        off_out = 2 * H + L * stride
+        rng = np.random.default_rng(seed)
+        p = np.zeros(off_out + H + 1)
+        p[0:H] = rng.standard_normal(H)
+        lim = np.sqrt(6.0 / (3 * H)) / 2
+        for l in range(L):
+            o = 2 * H + l * stride
+            p[o:o + 2 * H * H] = rng.uniform(-lim, lim, 2 * H * H)
+        p[off_out:off_out + H] = out_scale * rng.standard_normal(H) / np.sqrt(H)
+        return p.astype(np.float32)
+
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32)
         self._check(self._lib.cfdxhip_mpnn_set_params(self._h, p, p.size))

@@ -96,6 +96,18 @@ __device__ __forceinline__ T ordered_sum(const T* p, uint32_t b, uint32_t e, T s
     return s;
 }
 
+// FP64 sums keep the reference's per-row order (bit-exact results); FP32
+// sums of long rows are split across lanes.
+template <class T>
+struct OrderedSum {
+    static constexpr bool value = true;
+};
+template <>
+struct OrderedSum<float> {
+    static constexpr bool value = false;
+};
+constexpr uint32_t kGroupRowNnz = 12;  // FP32: 16 lanes per row above this mean row length
+
 template <class T, class Init, class Epi>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __restrict__ blocks, uint32_t nb,
                                                              const uint32_t* __restrict__ rowp,
@@ -103,6 +115,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
                                                              const T* __restrict__ vals, const T* __restrict__ x,
                                                              Init init, Epi epi) {
 #pragma clang fp contract(off)
+    constexpr bool kOrdered = OrderedSum<T>::value;
     __shared__ T prod[kStreamNnz];
     __shared__ double red[4];
     const uint32_t b = xcd_block(blockIdx.x, nb);
@@ -130,11 +143,26 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
             if (k < k1) prod[k - k0] = vi[j] * x[ci[j]];
         }
         __syncthreads();
-        for (uint32_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
-            const T s = ordered_sum(prod, rowp[row] - k0, rowp[row + 1] - k0, init(row));
-            part += epi(row, s);
+        if (kOrdered || (k1 - k0) < kGroupRowNnz * (r1 - r0)) {
+            for (uint32_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
+                const T s = ordered_sum(prod, rowp[row] - k0, rowp[row + 1] - k0, init(row));
+                part += epi(row, s);
+            }
+        } else {
+            // Long rows (FP32 only, no ordering contract): 16 lanes per row,
+            // strided partial sums combined by a shuffle tree.
+            const uint32_t g = threadIdx.x >> 4, lane = threadIdx.x & 15;
+            for (uint32_t row0 = r0; row0 < r1; row0 += kBlock / 16) {
+                const uint32_t row = row0 + g;
+                T s = T(0);
+                if (row < r1)
+                    for (uint32_t k = rowp[row] - k0 + lane; k < rowp[row + 1] - k0; k += 16) s += prod[k];
+#pragma unroll
+                for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+                if (row < r1 && lane == 0) part += epi(row, init(row) + s);
+            }
         }
-    } else {
+    } else if (kOrdered) {
         // One long row: stream it in chunks, a single thread keeps the
         // sequential sum.
         T s = init(r0);
@@ -146,6 +174,16 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
             __syncthreads();
         }
         if (threadIdx.x == 0) part = epi(r0, s);
+    } else {
+        // One long row, FP32: per-thread strided sums, then a block tree.
+        __shared__ T tred[kBlock / 64];
+        T s = T(0);
+        for (uint32_t k = k0 + threadIdx.x; k < k1; k += kBlock) s += vals[k] * x[cols[k]];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if ((threadIdx.x & 63) == 0) tred[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) part = epi(r0, init(r0) + (((tred[0] + tred[1]) + tred[2]) + tred[3]));
     }
     if (Epi::kPartial) {
         const double bs = block_sum(part, red);

@@ -2,9 +2,10 @@
 // gfx950). Layout and algorithm are described in mpnn.hpp.
 //
 // Tiling: every node kernel works on tiles of 64 nodes per 256-thread
-// workgroup. The gather phase gives each node 32 lanes (one per channel); the
-// dense phase gives each wavefront 16 nodes and runs 16x16x16 BF16 MFMAs on
-// LDS tiles. Parameter gradients are written as per-tile partials (one row of
+// workgroup. The gather phase gives each node 4 lanes of 8 channels (16-byte
+// BF16 rows), with neighbour loads issued in batches of eight; the dense
+// phase gives each wavefront 16 nodes and runs 16x16x16 BF16 MFMAs on LDS
+// tiles. Parameter gradients are written as per-tile partials (one row of
 // kMpnnParams floats per tile) and summed in tile order: no atomics, the
 // result does not depend on scheduling.
 #include "mpnn.hpp"
@@ -59,6 +60,67 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return uint16_t(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+__device__ __forceinline__ uint32_t pack2(float a, float b) { return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16); }
+
+// Eight channels [8q, 8q + 8) of node j: BF16 hidden states or FP32 rows.
+struct LoadBf8 {
+    const uint16_t* h;
+    uint32_t q;
+    __device__ __forceinline__ void operator()(uint32_t j, float* v) const {
+        const uint4 u = *reinterpret_cast<const uint4*>(h + size_t(j) * kHid + 8 * q);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v[2 * t] = __uint_as_float(w[t] << 16);
+            v[2 * t + 1] = __uint_as_float(w[t] & 0xffff0000u);
+        }
+    }
+};
+struct LoadF8 {
+    const float* g;
+    uint32_t q;
+    __device__ __forceinline__ void operator()(uint32_t j, float* v) const {
+        const float4 a = *reinterpret_cast<const float4*>(g + size_t(j) * kHid + 8 * q);
+        const float4 b = *reinterpret_cast<const float4*>(g + size_t(j) * kHid + 8 * q + 4);
+        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+    }
+};
+
+// m[c] = sum_{j != i} w_ij x_j[8q + c], w_ij = (-a_ij s_i) s_j, in entry
+// order. Neighbour indices, then the neighbours' rows, are loaded in batches
+// of eight so that their latencies overlap.
+template <class Load>
+__device__ __forceinline__ void gather8(uint32_t i, const uint32_t* __restrict__ rowp,
+                                        const uint32_t* __restrict__ cols, const float* __restrict__ vals,
+                                        const float* __restrict__ s, const Load& ld, float* m) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) m[c] = 0.0f;
+    const float si = s[i];
+    const uint32_t k0 = rowp[i], k1 = rowp[i + 1];
+    for (uint32_t k = k0; k < k1; k += 8) {
+        uint32_t jj[8];
+        float ww[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool ok = k + u < k1;
+            jj[u] = ok ? cols[k + u] : i;
+            ww[u] = ok ? vals[k + u] : 0.0f;
+        }
+        float sj[8], hv[8][8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            sj[u] = s[jj[u]];
+            ld(jj[u], hv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (jj[u] == i) continue;
+            const float w = (-ww[u] * si) * sj[u];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) m[c] += w * hv[u][c];
+        }
+    }
+}
 
 // D (16x16, FP32) += A (16 x K) * B (K x 16), A row-major in LDS with stride
 // lda, B supplied transposed (Bt[n][k], stride ldb). Lane l feeds
@@ -135,23 +197,25 @@ __global__ __launch_bounds__(kBlock) void layer_kernel(uint32_t n, const uint32_
     const int tid = threadIdx.x;
     const uint32_t node0 = blockIdx.x * kTile;
     for (int e = tid; e < kHid * 2 * kHid; e += kBlock) Wt[(e >> 6) * kLd + (e & 63)] = wt_g[e];
-    const int c = tid & 31, sub = tid >> 5;
-    for (int il = sub; il < kTile; il += 8) {
+    // 4 lanes per node, 8 channels per lane.
+    const int il = tid >> 2, q = tid & 3;
+    {
         const uint32_t i = node0 + il;
-        uint16_t hv = 0;
-        float m = 0.0f;
+        uint4 hv = {0u, 0u, 0u, 0u};
+        float m[8];
         if (i < n) {
-            hv = hin[size_t(i) * kHid + c];
-            const float si = s[i];
-            for (uint32_t k = rowp[i]; k < rowp[i + 1]; ++k) {
-                const uint32_t j = cols[k];
-                if (j == i) continue;
-                const float w = (-vals[k] * si) * s[j];
-                m += w * bf2f(hin[size_t(j) * kHid + c]);
-            }
+            hv = *reinterpret_cast<const uint4*>(hin + size_t(i) * kHid + 8 * q);
+            gather8(i, rowp, cols, vals, s, LoadBf8{hin, uint32_t(q)}, m);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) m[c] = 0.0f;
         }
-        X[il * kLd + c] = hv;
-        X[il * kLd + kHid + c] = f2bf(m);
+        uint2* xh = reinterpret_cast<uint2*>(X + il * kLd + 8 * q);
+        xh[0] = make_uint2(hv.x, hv.y);
+        xh[1] = make_uint2(hv.z, hv.w);
+        uint2* xm = reinterpret_cast<uint2*>(X + il * kLd + kHid + 8 * q);
+        xm[0] = make_uint2(pack2(m[0], m[1]), pack2(m[2], m[3]));
+        xm[1] = make_uint2(pack2(m[4], m[5]), pack2(m[6], m[7]));
     }
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
@@ -169,9 +233,13 @@ __global__ __launch_bounds__(kBlock) void layer_kernel(uint32_t n, const uint32_
         }
     }
     __syncthreads();
-    for (int il = sub; il < kTile; il += 8) {
+    {
         const uint32_t i = node0 + il;
-        if (i < n) hout[size_t(i) * kHid + c] = f2bf(Y[il * (kHid + 1) + c]);
+        if (i < n) {
+            const float* y = Y + il * (kHid + 1) + 8 * q;
+            *reinterpret_cast<uint4*>(hout + size_t(i) * kHid + 8 * q) =
+                make_uint4(pack2(y[0], y[1]), pack2(y[2], y[3]), pack2(y[4], y[5]), pack2(y[6], y[7]));
+        }
     }
     if (kTrain && tid < kTile && node0 + tid < n) {
         uint32_t bits = 0;
@@ -180,17 +248,23 @@ __global__ __launch_bounds__(kBlock) void layer_kernel(uint32_t n, const uint32_
     }
 }
 
-// e_i = rho s_i (h_i . w_out + b_out); thread per node.
+// e_i = rho s_i (h_i . w_out + b_out); 4 lanes per node, 8 channels each.
 __global__ __launch_bounds__(kBlock) void decode_kernel(uint32_t n, const uint16_t* __restrict__ h,
                                                        const float* __restrict__ s, const double* __restrict__ scal,
                                                        const float* __restrict__ p, float* __restrict__ e) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float* w = p + kOffWout;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = t >> 2, q = t & 3;
+    const float* w = p + kOffWout + 8 * q;
     float y = 0.0f;
-    for (int c = 0; c < kHid; ++c) y += bf2f(h[size_t(i) * kHid + c]) * w[c];
-    y += w[kHid];
-    e[i] = rho_of(scal, n) * s[i] * y;
+    if (i < n) {
+        float v[8];
+        LoadBf8{h, q}(i, v);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) y += v[c] * w[c];
+    }
+    y += __shfl_xor(y, 1, 4);
+    y += __shfl_xor(y, 2, 4);
+    if (i < n && q == 0) e[i] = rho_of(scal, n) * s[i] * (y + p[kOffWout + kHid]);
 }
 
 // Sum of acc over the 8 node slots of a 64-node tile, per channel, into
@@ -246,38 +320,33 @@ __global__ __launch_bounds__(kBlock) void layer_bwd_kernel(
     __shared__ __align__(16) uint16_t GZT[kHid * kLd];      // [c][node]
     __shared__ __align__(16) uint16_t W[2 * kHid * kLd32];  // [f][c]
     __shared__ float G[kTile * (2 * kHid + 1)];             // dL/dX [node][f]
-    __shared__ float red[8 * kHid];
+    __shared__ float GF[kTile * (kHid + 1)];                // FP32 gz [node][c]
     const int tid = threadIdx.x;
     const uint32_t node0 = blockIdx.x * kTile;
     for (int e = tid; e < 2 * kHid * kHid; e += kBlock) W[(e >> 5) * kLd32 + (e & 31)] = w_g[e];
-    const int c = tid & 31, sub = tid >> 5;
-    float ghv[8];
-    float ab = 0.0f;
+    // 4 lanes per node, 8 channels per lane.
+    const int il = tid >> 2, q = tid & 3;
+    const uint32_t i = node0 + il;
+    float ghv[8], m[8], hv[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int il = sub + 8 * t;
-        const uint32_t i = node0 + il;
-        uint16_t hv = 0;
-        float m = 0.0f, g = 0.0f, gz = 0.0f;
-        if (i < n) {
-            hv = hprev[size_t(i) * kHid + c];
-            const float si = s[i];
-            for (uint32_t k = rowp[i]; k < rowp[i + 1]; ++k) {
-                const uint32_t j = cols[k];
-                if (j == i) continue;
-                const float w = (-vals[k] * si) * s[j];
-                m += w * bf2f(hprev[size_t(j) * kHid + c]);
-            }
-            g = gh[size_t(i) * kHid + c];
-            gz = ((mask[i] >> c) & 1u) ? g : 0.0f;
-        }
-        ghv[t] = g;
-        ab += gz;
-        XT[c * kLd + il] = hv;
-        XT[(kHid + c) * kLd + il] = f2bf(m);
+    for (int c = 0; c < 8; ++c) ghv[c] = m[c] = hv[c] = 0.0f;
+    uint32_t mk = 0;
+    if (i < n) {
+        LoadBf8{hprev, uint32_t(q)}(i, hv);
+        gather8(i, rowp, cols, vals, s, LoadBf8{hprev, uint32_t(q)}, m);
+        LoadF8{gh, uint32_t(q)}(i, ghv);
+        mk = mask[i] >> (8 * q);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int f = 8 * q + c;
+        const float gz = ((mk >> c) & 1u) ? ghv[c] : 0.0f;
+        XT[f * kLd + il] = f2bf(hv[c]);  // exact: hv holds BF16 values
+        XT[(kHid + f) * kLd + il] = f2bf(m[c]);
         const uint16_t gzb = f2bf(gz);
-        GZ[il * kLd32 + c] = gzb;
-        GZT[c * kLd + il] = gzb;
+        GZ[il * kLd32 + f] = gzb;
+        GZT[f * kLd + il] = gzb;
+        GF[il * (kHid + 1) + f] = gz;
     }
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
@@ -300,16 +369,22 @@ __global__ __launch_bounds__(kBlock) void layer_bwd_kernel(
         for (int j = 0; j < 4; ++j) out[(wave * 16 + 4 * (lane >> 4) + j) * kHid + nt * 16 + (lane & 15)] = acc[j];
     }
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int il = sub + 8 * t;
-        const uint32_t i = node0 + il;
-        if (i >= n) continue;
-        gh2[size_t(i) * kHid + c] = ghv[t] + G[il * (2 * kHid + 1) + c];
-        gm[size_t(i) * kHid + c] = G[il * (2 * kHid + 1) + kHid + c];
+    if (i < n) {
+        const float* gr = G + il * (2 * kHid + 1);
+        float4* o2 = reinterpret_cast<float4*>(gh2 + size_t(i) * kHid + 8 * q);
+        o2[0] = make_float4(ghv[0] + gr[8 * q], ghv[1] + gr[8 * q + 1], ghv[2] + gr[8 * q + 2], ghv[3] + gr[8 * q + 3]);
+        o2[1] = make_float4(ghv[4] + gr[8 * q + 4], ghv[5] + gr[8 * q + 5], ghv[6] + gr[8 * q + 6],
+                            ghv[7] + gr[8 * q + 7]);
+        float4* om = reinterpret_cast<float4*>(gm + size_t(i) * kHid + 8 * q);
+        om[0] = make_float4(gr[kHid + 8 * q], gr[kHid + 8 * q + 1], gr[kHid + 8 * q + 2], gr[kHid + 8 * q + 3]);
+        om[1] = make_float4(gr[kHid + 8 * q + 4], gr[kHid + 8 * q + 5], gr[kHid + 8 * q + 6], gr[kHid + 8 * q + 7]);
     }
-    const float tb = tile_channel_sum(ab, red);
-    if (tid < kHid) out[2 * kHid * kHid + tid] = tb;
+    // dL/db: column sums of the FP32 gz tile in node order.
+    if (tid < kHid) {
+        float tb = 0.0f;
+        for (int r = 0; r < kTile; ++r) tb += GF[r * (kHid + 1) + tid];
+        out[2 * kHid * kHid + tid] = tb;
+    }
 }
 
 // gh_out = gh2 + sum_j w_ij gm_j (the mixing is symmetric for symmetric A).
@@ -318,18 +393,15 @@ __global__ __launch_bounds__(kBlock) void mix_t_kernel(uint32_t n, const uint32_
                                                       const float* __restrict__ vals, const float* __restrict__ s,
                                                       const float* __restrict__ gm, const float* __restrict__ gh2,
                                                       float* __restrict__ gh_out) {
-    const uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const uint32_t i = uint32_t(t >> 5), c = uint32_t(t & 31);
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = t >> 2, q = t & 3;
     if (i >= n) return;
-    const float si = s[i];
-    float acc = 0.0f;
-    for (uint32_t k = rowp[i]; k < rowp[i + 1]; ++k) {
-        const uint32_t j = cols[k];
-        if (j == i) continue;
-        const float w = (-vals[k] * si) * s[j];
-        acc += w * gm[size_t(j) * kHid + c];
-    }
-    gh_out[size_t(i) * kHid + c] = gh2[size_t(i) * kHid + c] + acc;
+    float acc[8], base[8];
+    gather8(i, rowp, cols, vals, s, LoadF8{gm, q}, acc);
+    LoadF8{gh2, q}(i, base);
+    float4* out = reinterpret_cast<float4*>(gh_out + size_t(i) * kHid + 8 * q);
+    out[0] = make_float4(base[0] + acc[0], base[1] + acc[1], base[2] + acc[2], base[3] + acc[3]);
+    out[1] = make_float4(base[4] + acc[4], base[5] + acc[5], base[6] + acc[6], base[7] + acc[7]);
 }
 
 // Encoder backward: partials of w_in (sum u gz) and b_in (sum gz).
@@ -535,7 +607,7 @@ void Mpnn::forward(DeviceAmg& amg, const float* r, float* e, bool keep) {
             layer_kernel<false><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], h_[l + 1], wt, bias,
                                                           nullptr);
     }
-    decode_kernel<<<grid_of(n), kBlock, 0, st>>>(n, h_[kMpnnLayers], s_, scal_, p_, e);
+    decode_kernel<<<grid_of(uint64_t(n) * 4), kBlock, 0, st>>>(n, h_[kMpnnLayers], s_, scal_, p_, e);
 }
 
 void Mpnn::backward(DeviceAmg& amg, const float* ge, float* grad_dev) {
@@ -556,12 +628,11 @@ void Mpnn::backward(DeviceAmg& amg, const float* ge, float* grad_dev) {
         gpart_cap_ = need;
     }
     decode_bwd_kernel<<<tiles, kBlock, 0, st>>>(n, ge, s_, scal_, h_[kMpnnLayers], p_, gh_, gpart_);
-    const uint32_t gn = uint32_t((uint64_t(n) * kHid + kBlock - 1) / kBlock);
     for (int l = kMpnnLayers - 1; l >= 0; --l) {
         const uint32_t off = 2 * kHid + l * kLayerStride;
         layer_bwd_kernel<<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], mask_[l + 1],
                                                    w_ + l * 2 * kHid * kHid, gh_, gh2_, gm_, gpart_, off);
-        mix_t_kernel<<<gn, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, gm_, gh2_, gh_);
+        mix_t_kernel<<<grid_of(uint64_t(n) * 4), kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, gm_, gh2_, gh_);
     }
     encode_bwd_kernel<<<tiles, kBlock, 0, st>>>(n, last_r_, s_, scal_, gh_, mask_[0], gpart_);
     grad_reduce_kernel<<<grid_of(kMpnnParams), kBlock, 0, st>>>(tiles, gpart_, grad_dev);

@@ -81,7 +81,18 @@ class Graph:
         return out
 
 
-def forward(g: Graph, p, r, keep=False):
+def bf16(a):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    x = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    x = (x + 0x7FFF + ((x >> 16) & 1)) & 0xFFFF0000
+    return x.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward(g: Graph, p, r, keep=False, emulate_bf16=False):
+    """e = N(r). With emulate_bf16 the values the device keeps in BF16 are
+    rounded at the same points: hidden states, the message half of each
+    layer input, and the layer weights; everything else stays float64."""
+    R = bf16 if emulate_bf16 else (lambda a: a)
     w_in, b_in, Ws, bs, w_out, b_out = unpack(p)
     rs = np.asarray(r, dtype=np.float64) * g.s
     rho = np.sqrt(np.mean(rs * rs))
@@ -89,12 +100,12 @@ def forward(g: Graph, p, r, keep=False):
         return (np.zeros(g.n), None) if keep else np.zeros(g.n)
     u = rs / rho
     z0 = u[:, None] * w_in[None, :] + b_in[None, :]
-    h = np.maximum(z0, 0.0)
-    tape = dict(u=u, rho=rho, z0=z0, hs=[h], zs=[], ms=[])
+    h = R(np.maximum(z0, 0.0))
+    tape = dict(u=u, rho=rho, z0=z0, hs=[h], zs=[], ms=[], R=R)
     for l in range(LAYERS):
-        m = g.mix(h)
-        z = np.concatenate([h, m], axis=1) @ Ws[l] + bs[l][None, :]
-        h = np.maximum(z, 0.0) + h
+        m = R(g.mix(h))
+        z = np.concatenate([h, m], axis=1) @ R(Ws[l]) + bs[l][None, :]
+        h = R(np.maximum(z, 0.0) + h)
         tape["ms"].append(m)
         tape["zs"].append(z)
         tape["hs"].append(h)
@@ -103,13 +114,19 @@ def forward(g: Graph, p, r, keep=False):
     return (e, tape) if keep else e
 
 
-def vjp(g: Graph, p, r, ge):
-    """Gradient of sum_i ge_i e_i(p) with respect to the flat parameters."""
+def vjp(g: Graph, p, r, ge, emulate_bf16=False):
+    """Gradient of sum_i ge_i e_i(p) with respect to the flat parameters.
+    With emulate_bf16 the upstream gradient of each layer's pre-activation is
+    rounded to BF16 where the device feeds it to the matrix cores; the exact
+    derivative of the BF16-rounded network is otherwise not defined, so this
+    mode is for comparing with the device, the plain mode for finite
+    differences."""
     w_in, b_in, Ws, bs, w_out, b_out = unpack(p)
-    e, t = forward(g, p, r, keep=True)
+    e, t = forward(g, p, r, keep=True, emulate_bf16=emulate_bf16)
     grad = np.zeros(NPARAMS)
     if t is None:
         return grad
+    R = t["R"]
     gy = np.asarray(ge, dtype=np.float64) * t["rho"] * g.s
     hL = t["hs"][-1]
     grad[OFF_WOUT:OFF_WOUT + H] = hL.T @ gy
@@ -119,10 +136,11 @@ def vjp(g: Graph, p, r, ge):
         o = 2 * H + l * LAYER_STRIDE
         hp, m, z = t["hs"][l], t["ms"][l], t["zs"][l]
         gz = gh * (z > 0.0)
+        GZ = R(gz)
         X = np.concatenate([hp, m], axis=1)
-        grad[o:o + 2 * H * H] = (X.T @ gz).ravel()
+        grad[o:o + 2 * H * H] = (X.T @ GZ).ravel()
         grad[o + 2 * H * H:o + LAYER_STRIDE] = gz.sum(0)
-        gX = gz @ Ws[l].T
+        gX = GZ @ R(Ws[l]).T
         gh = gh + gX[:, :H] + g.mix(gX[:, H:])  # A symmetric: mix^T = mix
     gz0 = gh * (t["z0"] > 0.0)
     grad[0:H] = t["u"] @ gz0

@@ -2,10 +2,12 @@
 (oracle/mpnn_oracle.py), training on solver-generated pairs, the fallback
 rule, and the ablation (mixed FGMRES with vs without the trained network).
 
-Tolerances: hidden states are BF16 (8-bit mantissa) and the dense products
-run on BF16 MFMA with FP32 accumulation, so the forward output is compared to
-the FP64 oracle at 3e-2 relative (2-norm) and the parameter gradient by
-cosine similarity >= 0.995 and norm ratio within 5 %.
+Tolerances. Hidden states are BF16 (8-bit mantissa) and the dense products
+run on BF16 MFMA with FP32 accumulation. Against the oracle that rounds to
+BF16 at the same points (emulate_bf16) the device must agree to FP32
+accumulation order: forward 2e-3 relative (2-norm), gradient cosine
+>= 0.9995 and norm ratio within 1 %. Against the plain FP64 oracle (the
+precision statement): forward 3e-2, gradient cosine >= 0.99.
 """
 import numpy as np
 import pytest
@@ -18,11 +20,19 @@ pytestmark = pytest.mark.gpu
 
 
 def _system(oracle, dims, seed, sigma=1.0):
-    return pressure_system(oracle, box_mesh(*dims, jitter=0.15, seed=seed), sigma=sigma, seed=seed)
+    """Pressure system scaled to a unit mean diagonal (the solution is
+    unchanged; the coarse LU's absolute pivot floor needs O(1) entries)."""
+    s = pressure_system(oracle, box_mesh(*dims, jitter=0.15, seed=seed), sigma=sigma, seed=seed)
+    rows = np.repeat(np.arange(s["n"]), np.diff(s["rowp"]))
+    scale = 1.0 / s["vals"][s["cols"] == rows].mean()
+    s["vals"] = s["vals"] * scale
+    s["b"] = s["b"] * scale
+    return s
 
 
 def _amg(s, **kw):
-    h = cx.NativeAMGPreconditioner(precision="fp32", smoother_fp32="chebyshev", **kw)
+    h = cx.NativeAMGPreconditioner(precision="fp32", smoother_fp32="chebyshev", prolongator_filter=0.2,
+                                   filter_keeps_aggregate=True, **kw)
     assert h.setup(s["n"], s["rowp"], s["cols"], s["vals"]), h.last_error()
     return h
 
@@ -43,8 +53,11 @@ def test_forward_matches_oracle(oracle, dims):
     rng = np.random.default_rng(2)
     for _ in range(2):
         r = rng.standard_normal(s["n"])
-        eo = M.forward(g, p, r)
         eg = net.forward(h, r)
+        eb = M.forward(g, p, r, emulate_bf16=True)
+        rel = np.linalg.norm(eg - eb) / np.linalg.norm(eb)
+        assert rel <= 2e-3, rel
+        eo = M.forward(g, p, r)
         rel = np.linalg.norm(eg - eo) / np.linalg.norm(eo)
         assert rel <= 3e-2, rel
     assert np.all(net.forward(h, np.zeros(s["n"])) == 0.0)
@@ -59,14 +72,15 @@ def test_vjp_matches_oracle(oracle):
     rng = np.random.default_rng(4)
     r = rng.standard_normal(s["n"])
     ge = rng.standard_normal(s["n"])
-    go = M.vjp(g, p, r, ge)
     gg = net.vjp(h, r, ge).astype(np.float64)
     blocks = [(0, 64)] + [(64 + 2080 * l, 64 + 2080 * (l + 1)) for l in range(4)] + [(M.OFF_WOUT, M.NPARAMS)]
-    for a, b in blocks:
-        x, y = gg[a:b], go[a:b]
-        cos = x @ y / (np.linalg.norm(x) * np.linalg.norm(y))
-        ratio = np.linalg.norm(x) / np.linalg.norm(y)
-        assert cos >= 0.995 and abs(ratio - 1) <= 0.05, (a, b, cos, ratio)
+    for emulate, cmin, rtol in ((True, 0.9995, 0.01), (False, 0.99, 0.05)):
+        go = M.vjp(g, p, r, ge, emulate_bf16=emulate)
+        for a, b in blocks:
+            x, y = gg[a:b], go[a:b]
+            cos = x @ y / (np.linalg.norm(x) * np.linalg.norm(y))
+            ratio = np.linalg.norm(x) / np.linalg.norm(y)
+            assert cos >= cmin and abs(ratio - 1) <= rtol, (emulate, a, b, cos, ratio)
 
 
 def test_adam_step_closed_form():
