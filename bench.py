@@ -8,12 +8,17 @@ reference cell eliminated, zero-mean N(0,1) right-hand side. The matrix is
 assembled on the GPU by the face-loop kernels.
 
 A step is one complete solve from x = 0 to ||b - A x||_2 / ||b||_2 <= 1e-8
-(FP64 true residual recomputed on the GPU):
-  * value:  FP64 FGMRES(30) outer iteration with the FP32 smoothed-aggregation
-            V-cycle as preconditioner (mixed precision),
-  * plain:  FP64 AMG-PCG with the FP64 V-cycle (the reference algorithm),
-both on the same GPU and the same hierarchy construction. Setup is timed
-separately and is not part of a step.
+(FP64 true residual recomputed on the GPU). Three solves per step, all on the
+same GPU and the same hierarchy:
+  * value:  learned + mixed -- FP64 FGMRES(30) whose flexible preconditioner
+            is the message-passing network stage ahead of one FP32
+            smoothed-aggregation V-cycle (Chebyshev smoothing), with the
+            fallback to the plain V-cycle when the stage stops paying,
+  * mixed:  the same FGMRES with the FP32 V-cycle alone,
+  * plain:  FP64 AMG-PCG with the FP64 V-cycle (the reference algorithm:
+            6 + 6 damped Jacobi sweeps with the certified omega).
+Setup and the network's training (on smaller boxes of the same family, on the
+GPU, before the timed steps) are timed separately and are not part of a step.
 
 The hierarchy is the reference's smoothed aggregation with its prolongator
 filter (amg_preconditioner.h:2020-2033) at 0.2, in this build's variant that
@@ -23,9 +28,10 @@ damping ~0.04) and its V-cycle is rejected at the first PCG step, while its
 filtered hierarchy empties whole P columns and is rejected at setup (zero
 coarse diagonal) for every filter tried (0.02 - 0.1).
 
-The JSON line also carries the roofline of the dominant kernel (the CSR-stream
-SpMV family; measured here as the finest-level FP64 SpMV with HIP events on
-the solver's stream) and the reference's CPU PCG timed on a bounded sample.
+The JSON line also carries the roofline of the dominant kernel family (the
+CSR-stream SpMV; measured as the finest-level FP32 SpMV with HIP events on the
+solver's stream, FP64 alongside) and the CPU FP64 AMG-PCG (the oracle's
+restatement of the reference algorithm, one core) timed on a bounded sample.
 """
 from __future__ import annotations
 
@@ -43,6 +49,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Solve s to 1e-8 rel residual, learned+mixed vs FP64 AMG-PCG; SpMV TB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the finest-level FP32 SpMV, from two rocprofv3 --pmc
+# passes (FETCH_SIZE, WRITE_SIZE) over this benchmark; FETCH_SIZE is scaled by
+# 2.0, the factor that makes it read the known size of a 1 GiB device copy.
+MEASURED_TRAFFIC = {
+    ("box256_walls30_ar1000_lognormal1.5", "fp32"): {
+        "bytes": 1170409681,
+        "source": "profiles/spmv_fp32_box256_hbm_traffic.md",
+    },
+}
 
 
 def log(*a):
@@ -107,8 +122,12 @@ def main():
     ap.add_argument("--spmv-repeat", type=int, default=20)
     ap.add_argument("--mixed-smoother", choices=["chebyshev", "jacobi"], default="chebyshev",
                     help="smoother of the FP32 V-cycle inside mixed FGMRES")
-    ap.add_argument("--cheb-degree", type=int, default=3)
-    ap.add_argument("--cheb-ratio", type=float, default=30.0)
+    ap.add_argument("--cheb-degree", type=int, default=4)
+    ap.add_argument("--cheb-ratio", type=float, default=60.0)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned stage (value = mixed solve)")
+    ap.add_argument("--train-n", type=int, default=48, help="training boxes are train_n^3")
+    ap.add_argument("--train-systems", type=int, default=12)
+    ap.add_argument("--train-steps", type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -143,34 +162,66 @@ def main():
     log(f"[bench] hierarchy: {len(levels)} levels {[(a, b) for a, b, _ in levels]} operator complexity {opcx:.2f}, "
         f"omega {h.smoother_omega():.3f}, setup {t_setup:.1f} s (host {hs:.1f} s, device {ds:.1f} s)")
     plain = mixed = h
-    t_setup_plain = t_setup_mixed = t_setup
+
+    # Learned stage: trained here, on the GPU, by the library's own kernels on
+    # solver-generated pairs from smaller boxes of the same family.
+    net, train_meta, t_train = None, None, 0.0
+    if not args.no_learned:
+        from cfdx_dev_amd.train_mpnn import train
+
+        t0 = time.perf_counter()
+        params, train_meta = train(n=args.train_n, systems=args.train_systems, steps=args.train_steps,
+                                   cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
+                                   log=lambda m: log("[train] " + m))
+        t_train = time.perf_counter() - t0
+        net = cx.LearnedStage(params=params)
+        log(f"[bench] learned stage trained in {t_train:.1f} s")
 
     def run_plain():
         x = np.zeros(s.n)
-        r = plain.solve_cg(s.b, x, 1000, args.tol)
-        return r
+        return plain.solve_cg(s.b, x, 1000, args.tol)
 
     def run_mixed():
         x = np.zeros(s.n)
-        r = mixed.solve_fgmres(s.b, x, 30, 1000, args.tol)
-        return r
+        return mixed.solve_fgmres(s.b, x, 30, 1000, args.tol)
+
+    def run_learned():
+        # The stage is dropped (plain V-cycle from then on) once the residual
+        # reduction per iteration over 3 iterations falls below 0.5.
+        net.set_fallback(0.5, 3)
+        x = np.zeros(s.n)
+        return mixed.solve_fgmres_learned(s.b, x, net, 30, 1000, args.tol)
 
     for _ in range(args.warmup):
         w1, w2 = run_plain(), run_mixed()
-        log(f"[bench] warmup: plain status {w1.status} {w1.iterations} it {w1.solve_seconds:.3f} s | mixed status "
-            f"{w2.status} {w2.iterations} it {w2.solve_seconds:.3f} s")
-    tp, tm, rp_last, rm_last = [], [], None, None
+        msg = (f"[bench] warmup: plain status {w1.status} {w1.iterations} it {w1.solve_seconds:.3f} s | mixed status "
+               f"{w2.status} {w2.iterations} it {w2.solve_seconds:.3f} s")
+        if net is not None:
+            w3 = run_learned()
+            msg += f" | learned status {w3.status} {w3.iterations} it {w3.solve_seconds:.3f} s"
+        log(msg)
+    tp, tm, tl = [], [], []
+    rp_last = rm_last = rl_last = None
     for k in range(args.steps):
         rp_last = run_plain()
         tp.append(rp_last.solve_seconds)
         rm_last = run_mixed()
         tm.append(rm_last.solve_seconds)
         assert rp_last.status == 0 and rm_last.status == 0, (rp_last, rm_last, h.last_error())
-        log(f"[bench] step {k}: plain PCG {rp_last.solve_seconds:.3f} s ({rp_last.iterations} it, "
-            f"rel {rp_last.residual_relative:.2e}) | mixed FGMRES {rm_last.solve_seconds:.3f} s "
-            f"({rm_last.iterations} it, rel {rm_last.residual_relative:.2e})")
+        msg = (f"[bench] step {k}: plain PCG {rp_last.solve_seconds:.3f} s ({rp_last.iterations} it, "
+               f"rel {rp_last.residual_relative:.2e}) | mixed FGMRES {rm_last.solve_seconds:.3f} s "
+               f"({rm_last.iterations} it, rel {rm_last.residual_relative:.2e})")
+        if net is not None:
+            rl_last = run_learned()
+            assert rl_last.status == 0, (rl_last, h.last_error())
+            tl.append(rl_last.solve_seconds)
+            msg += (f" | learned+mixed {rl_last.solve_seconds:.3f} s ({rl_last.iterations} it, "
+                    f"stage {rl_last.extra['learned_stage_applications']} applications, "
+                    f"{rl_last.extra['learned_stage_fallbacks']} fallbacks)")
+        log(msg)
     t_plain = float(np.mean(tp))
     t_mixed = float(np.mean(tm))
+    t_value = float(np.mean(tl)) if tl else t_mixed
 
     # ---- SpMV roofline (finest level, FP64 and FP32), HIP events on the solver stream
     import ctypes as C
@@ -234,18 +285,33 @@ def main():
         cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.filter)
         log(f"[bench] CPU baseline ({time.perf_counter() - t0:.1f} s): {cpu}")
 
+    wl_traffic = MEASURED_TRAFFIC.get((s.name, "fp32"))
+    learned_info = None
+    if net is not None:
+        learned_info = {
+            "iterations": int(rl_last.iterations),
+            "stage_applications": int(rl_last.extra["learned_stage_applications"]),
+            "fallbacks": int(rl_last.extra["learned_stage_fallbacks"]),
+            "fallback_rule": "stage dropped for the rest of the solve once the residual reduction per "
+                             "iteration, 1 - (|r_k|/|r_k-3|)^(1/3), falls below 0.5",
+            "training": {"boxes": f"{args.train_n}^3 x {args.train_systems} (seeds 1000..)",
+                         "steps_per_system": args.train_steps, "pairs_per_system": len(train_meta["pair_iterations"]),
+                         "seconds": round(t_train, 2),
+                         "loss_first": train_meta["history"][0]["loss_first"],
+                         "loss_last": train_meta["history"][-1]["loss_last"]},
+        }
     out = {
         "metric": METRIC,
-        "value": round(t_mixed, 6),
+        "value": round(t_value, 6),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_mixed * 1e3, 3),
+        "ms_per_step": round(t_value * 1e3, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 outer / f32 hierarchy",
+        "dtype": "f64 outer / f32 hierarchy / bf16 learned stage",
         "data": "synthetic",
         "config": {
             "workload": s.name,
@@ -253,17 +319,21 @@ def main():
             "unknowns": int(s.n),
             "nnz": int(nnz),
             "tolerance": args.tol,
-            "solver": "FGMRES(30) FP64 outer + FP32 SA V-cycle (" + (
-                f"Chebyshev degree {args.cheb_degree}, ratio {args.cheb_ratio:g}"
-                if args.mixed_smoother == "chebyshev" else "6+6 damped Jacobi") + ")",
-            "plain": "PCG FP64 + FP64 SA V-cycle, true residual every iteration",
-            "learned_stage": False,
+            "solver": ("FGMRES(30) FP64 outer + " + ("[MPNN stage -> " if net is not None else "[") +
+                       "FP32 SA V-cycle (" + (
+                           f"Chebyshev degree {args.cheb_degree}, ratio {args.cheb_ratio:g}"
+                           if args.mixed_smoother == "chebyshev" else "6+6 damped Jacobi") + ")]"),
+            "plain": "PCG FP64 + FP64 SA V-cycle (6+6 damped Jacobi, certified omega), true residual every iteration",
+            "learned_stage": net is not None,
             "parallelism": "single GPU",
         },
         "plain_fp64_pcg_s": round(t_plain, 6),
+        "mixed_fgmres_s": round(t_mixed, 6),
         "plain_iterations": int(rp_last.iterations),
         "mixed_iterations": int(rm_last.iterations),
-        "speedup_vs_plain": round(t_plain / t_mixed, 3),
+        "learned": learned_info,
+        "speedup_vs_plain": round(t_plain / t_value, 3),
+        "speedup_mixed_only_vs_plain": round(t_plain / t_mixed, 3),
         "setup_s": {"total": round(t_setup, 3), "host": round(hs, 3), "device": round(ds, 3)},
         "operator_complexity": round(opcx, 3),
         "smoother_omega": h.smoother_omega(),
@@ -272,17 +342,20 @@ def main():
         "amg_levels": [[int(a), int(b)] for a, b, _ in levels],
         "spmv_fp64_TBps": round(gbs64 / 1e3, 4),
         "spmv_fp32_TBps": round(gbs32 / 1e3, 4),
+        "spmv_fp64_frac_of_peak": round(gbs64 / HBM_PEAK_GBS, 4),
         "device_copy_TBps": round(copy_gbs / 1e3, 4),
         "roofline": {
             "bound": "hbm",
-            "achieved": round(gbs64, 1),
+            "achieved": round(gbs32, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "spmv_stream_kernel<double,InitZero,EpiStore> (finest level)",
-            "bytes_per_launch": int(b64),
-            "us_per_launch": round(t64 * 1e6, 2),
+            "frac": round(gbs32 / HBM_PEAK_GBS, 4),
+            "traffic": None if wl_traffic is None else wl_traffic["bytes"],
+            "traffic_source": None if wl_traffic is None else wl_traffic["source"],
+            "kernel": "spmv_stream_kernel<float,InitZero,EpiStore> (finest level; the FP32 CSR-stream family "
+                      "carries the mixed solve)",
+            "bytes_per_launch": int(b32),
+            "us_per_launch": round(t32 * 1e6, 2),
         },
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "cpu_info": None if cpu is None else {"cpu": cpu["cpu"], "nproc": cpu["nproc"], "setup_s": cpu["setup_s"],

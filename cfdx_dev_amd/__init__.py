@@ -194,6 +194,15 @@ class NativeAMGPreconditioner:
             raise IndexError(level)
         return agg, rp, cols, vals
 
+    def level_kernel_seconds(self, level: int, precision: str = "fp64", kind: str = "spmv", repeat: int = 10):
+        """Seconds per launch of a level's SpMV ("spmv") or fused Jacobi sweep ("jacobi")."""
+        v = C.c_double()
+        st = self._lib.cfdxhip_amg_level_kernel_seconds(self._h, level, PRECISIONS[precision],
+                                                        {"spmv": 0, "jacobi": 1}[kind], repeat, C.byref(v))
+        if st != 0:
+            raise RuntimeError(self.last_error())
+        return v.value
+
     def set_chebyshev(self, degree: int, ratio: float = 30.0) -> bool:
         self.last_status = self._lib.cfdxhip_amg_set_chebyshev(self._h, degree, ratio)
         return self.last_status == 0
@@ -459,5 +468,7 @@ class LearnedStage:
         self._check(self._lib.cfdxhip_mpnn_train(self._h, amg._h, steps, lr, losses))
         return losses
 
-    def set_fallback(self, reference_rate, ratio=0.5, window=5):
-        self._check(self._lib.cfdxhip_mpnn_set_fallback(self._h, reference_rate, ratio, window))
+    def set_fallback(self, min_reduction=0.5, window=3):
+        """Drop the stage once the per-iteration residual reduction over
+        `window` iterations falls below min_reduction (negative: never)."""
+        self._check(self._lib.cfdxhip_mpnn_set_fallback(self._h, min_reduction, window))

@@ -136,6 +136,8 @@ def load():
     _sig(lib, "cfdxhip_amg_smoother_omega", st, [vp, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_level_lambda_max", st, [vp, C.c_size_t, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_set_chebyshev", st, [vp, C.c_uint32, C.c_double])
+    _sig(lib, "cfdxhip_amg_level_kernel_seconds", st,
+         [vp, C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_mpnn_param_count", C.c_size_t, [])
     _sig(lib, "cfdxhip_mpnn_create", st, [C.c_int32, C.POINTER(vp)])
     _sig(lib, "cfdxhip_mpnn_destroy", None, [vp])
@@ -148,7 +150,7 @@ def load():
     _sig(lib, "cfdxhip_mpnn_collect_pairs", st,
          [vp, vp, f64p, u64p, C.c_uint32, C.c_double, C.POINTER(C.c_uint32), C.POINTER(Result)])
     _sig(lib, "cfdxhip_mpnn_train", st, [vp, vp, C.c_uint32, C.c_float, f64p])
-    _sig(lib, "cfdxhip_mpnn_set_fallback", st, [vp, C.c_double, C.c_double, C.c_uint32])
+    _sig(lib, "cfdxhip_mpnn_set_fallback", st, [vp, C.c_double, C.c_uint32])
     _sig(lib, "cfdxhip_amg_solve_fgmres_learned", st,
          [vp, vp, f64p, f64p, C.c_int, C.c_size_t, C.c_double, C.POINTER(Result)])
     _sig(lib, "cfdxhip_amg_solve_fgmres_learned_device", st,

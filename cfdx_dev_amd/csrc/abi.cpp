@@ -452,6 +452,19 @@ cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* hs, doubl
     return CFDXHIP_OK;
 }
 
+cfdxhip_status cfdxhip_amg_level_kernel_seconds(cfdxhip_amg* h, size_t level, uint32_t precision, uint32_t kind,
+                                                uint32_t repeat, double* seconds) {
+    if (!h || !seconds) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready || !h->dev) return fail(h, CFDXHIP_ERR_NOT_READY, "not set up on a device");
+    return guarded(h, [&]() {
+        hip_check(hipSetDevice(h->opt.device), "hipSetDevice");
+        std::string err;
+        if (!h->dev->level_kernel_seconds(uint32_t(level), precision, kind, repeat, *seconds, err))
+            return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, err);
+        return CFDXHIP_OK;
+    });
+}
+
 cfdxhip_status cfdxhip_amg_spmv_device(cfdxhip_amg* h, uint32_t precision, const void* x, void* y, uint32_t repeat,
                                        double* seconds) {
     if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
@@ -855,11 +868,10 @@ cfdxhip_status cfdxhip_mpnn_train(cfdxhip_mpnn* m, cfdxhip_amg* h, uint32_t step
     });
 }
 
-cfdxhip_status cfdxhip_mpnn_set_fallback(cfdxhip_mpnn* m, double reference_rate, double ratio, uint32_t window) {
-    if (!m || !(reference_rate >= 0.0) || !std::isfinite(reference_rate) || !(ratio >= 0.0) || window == 0)
+cfdxhip_status cfdxhip_mpnn_set_fallback(cfdxhip_mpnn* m, double min_reduction, uint32_t window) {
+    if (!m || !(min_reduction <= 1.0) || window == 0 || window > 64)
         return mfail(m, CFDXHIP_ERR_INVALID_ARGUMENT, "bad fallback arguments");
-    m->m->reference_rate = reference_rate;
-    m->m->fallback_ratio = ratio;
+    m->m->fallback_min_reduction = min_reduction;
     m->m->fallback_window = window;
     return CFDXHIP_OK;
 }
@@ -880,7 +892,7 @@ static cfdxhip_status learned_common(cfdxhip_amg* h, cfdxhip_mpnn* m, const doub
         std::string err;
         LearnedControl lc;
         lc.stage = m->m.get();
-        lc.min_rate = m->m->fallback_ratio * m->m->reference_rate;
+        lc.min_reduction = m->m->fallback_min_reduction;
         lc.window = m->m->fallback_window;
         bool ok;
         if (device) {

@@ -37,7 +37,10 @@ namespace cfdxhip {
 
 namespace {
 
-constexpr uint32_t kGsStride = 1024;  // partial stride of the Gram-Schmidt reductions
+constexpr uint32_t kGsStride = 4096;  // partial stride (and grid cap) of the Gram-Schmidt reductions
+// Second Gram-Schmidt pass when the first leaves |w'| <= |w| / sqrt(2)
+// (Daniel, Gragg, Kaufman and Stewart 1976; "twice is enough").
+constexpr double kDgks = 0.70710678118654752;
 constexpr uint32_t kScalars = 512;    // device scalar slots
 
 template <class T>
@@ -982,8 +985,8 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
         if (!stage_on || hist.size() <= learned->window) return;
         const size_t k = hist.size() - 1;
         const double a = hist[k - learned->window], b2 = hist[k];
-        const double rate = (a > 0.0 && b2 > 0.0) ? std::log10(a / b2) / double(learned->window) : 0.0;
-        if (rate < learned->min_rate) {
+        const double red = a > 0.0 ? 1.0 - std::pow(b2 / a, 1.0 / double(learned->window)) : 0.0;
+        if (red < learned->min_reduction) {
             stage_on = false;
             ++learned->fallbacks;
         }
@@ -1028,14 +1031,11 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
             // w = A z_j
             launch_spmv<double>(L0.A, zj, InitZero<double>{}, EpiStore<double>{w}, stream_);
             // first CGS pass: h1 = V^T w, |w|^2 (slots 16..)
+            // Classical Gram-Schmidt with a conditional second pass (DGKS):
+            // h1 = V^T w and |w|^2 (slots 16..), w -= V h1 and |w'|^2 (304).
             gs_dot_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, w, partials_, kGsStride);
             reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 16);
-            // w -= V h1; h2 = V^T w (slots 160..)
-            gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 16, w, partials_, kGsStride, true);
-            reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 160);
-            // w -= V h2; |w|^2 (slot 304)
-            gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 160, w, partials_, kGsStride,
-                                                        false);
+            gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 16, w, partials_, kGsStride, false);
             reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, 1, scal_ + 304);
             HIP_OK(hipMemcpyAsync(h_scal_, scal_, 305 * sizeof(double), hipMemcpyDeviceToHost, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
@@ -1045,8 +1045,21 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
                 HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
                 return finish(2, its, best);
             }
-            for (uint32_t i = 0; i <= j; ++i) Hr(i, j) = hh[16 + i] + hh[160 + i];
-            const double hn = std::sqrt(hh[304]);
+            for (uint32_t i = 0; i <= j; ++i) Hr(i, j) = hh[16 + i];
+            double hn = std::sqrt(hh[304]);
+            if (!(hn > kDgks * an)) {
+                // Cancellation: second pass h2 = V^T w (slots 160..), w -= V h2.
+                ++st.reorthogonalisations;
+                gs_dot_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, w, partials_, kGsStride);
+                reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 160);
+                gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 160, w, partials_, kGsStride,
+                                                            false);
+                reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, 1, scal_ + 304);
+                HIP_OK(hipMemcpyAsync(h_scal_, scal_, 305 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+                HIP_OK(hipStreamSynchronize(stream_));
+                for (uint32_t i = 0; i <= j; ++i) Hr(i, j) += hh[160 + i];
+                hn = std::sqrt(hh[304]);
+            }
             if (!std::isfinite(hn)) {
                 HIP_OK(hipMemcpyAsync(x, bestx, size_t(n) * 8, hipMemcpyDeviceToDevice, stream_));
                 return finish(2, its, best);
@@ -1144,6 +1157,48 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
 }
 
 // ------------------------------------------------------------ SpMV API --
+
+bool DeviceAmg::level_kernel_seconds(uint32_t level, uint32_t precision, uint32_t kind, uint32_t repeat,
+                                     double& sec, std::string& err) {
+    if (level >= lv_.size() || precision > 1 || kind > 1) {
+        err = "invalid level or kernel";
+        return false;
+    }
+    if ((precision == 0 && !has64_) || (precision == 1 && !has32_)) {
+        err = "hierarchy precision not built";
+        return false;
+    }
+    DevLevel& L = lv_[level];
+    if (repeat == 0) repeat = 1;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        T* const* xs = x_of<T>(L);
+        T* b = b_of<T>(L);
+        fill_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, T(1), xs[0]);
+        fill_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, T(1), b);
+        HIP_OK(hipEventRecord(e0, stream_));
+        for (uint32_t k = 0; k < repeat; ++k) {
+            if (kind == 0)
+                launch_spmv<T>(L.A, xs[0], InitZero<T>{}, EpiStore<T>{xs[1]}, stream_);
+            else
+                launch_spmv<T>(L.A, xs[0], InitZero<T>{}, EpiJacobi<T>{b, xs[0], inv_of<T>(L), xs[1], T(omega_)},
+                               stream_);
+        }
+        HIP_OK(hipEventRecord(e1, stream_));
+    };
+    if (precision == 0) run(double{});
+    else run(float{});
+    HIP_OK(hipStreamSynchronize(stream_));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sec = double(ms) * 1e-3 / repeat;
+    return true;
+}
 
 bool DeviceAmg::spmv_caller(uint32_t precision, const void* x_caller, void* y_caller, uint32_t repeat,
                             double& sec, std::string& err) {

@@ -56,6 +56,7 @@ struct SolveStats {
     double max_gap = 0.0, min_true = 0.0, max_true = 0.0;
     uint64_t first_replacement = 0, last_replacement = 0;
     uint64_t pc_applications = 0;
+    uint64_t reorthogonalisations = 0;  // FGMRES second Gram-Schmidt passes
     double seconds = 0.0;
 };
 
@@ -78,11 +79,11 @@ public:
 // Learned-stage control of one mixed FGMRES solve. The stage is dropped for
 // the rest of the solve (plain V-cycle, counted as a fallback) once the
 // per-iteration residual reduction over the last `window` iterations,
-// log10(|g_{k-window}| / |g_k|) / window from the Givens estimates, drops
-// below min_rate.
+// 1 - (|g_k| / |g_{k-window}|)^(1/window) from the Givens estimates, drops
+// below min_reduction.
 struct LearnedControl {
     PreconditionerStage* stage = nullptr;
-    double min_rate = 0.0;
+    double min_reduction = 0.5;
     uint32_t window = 5;
     uint64_t applications = 0;
     uint64_t fallbacks = 0;
@@ -131,6 +132,10 @@ public:
     // r = b - A x on the finest level, FP32, solve ordering.
     void residual32(const float* b, const float* x, float* r);
     const uint32_t* device_perm0() const { return d_perm0_; }
+    // Seconds per launch of a level's SpMV (kind 0: y = A x) or fused Jacobi
+    // sweep (kind 1) on the level's own buffers (x = b = 1), HIP events.
+    bool level_kernel_seconds(uint32_t level, uint32_t precision, uint32_t kind, uint32_t repeat, double& sec,
+                              std::string& err);
     // y = A x on caller-ordered device vectors; returns seconds per launch.
     bool spmv_caller(uint32_t precision, const void* x, void* y, uint32_t repeat, double& sec, std::string& err);
 

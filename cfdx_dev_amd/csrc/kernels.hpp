@@ -20,7 +20,10 @@
 namespace cfdxhip {
 
 constexpr int kBlock = 256;
-constexpr int kStreamNnz = 2048;  // staged products per row block
+#ifndef CFDX_STREAM_NNZ
+#define CFDX_STREAM_NNZ 2048
+#endif
+constexpr int kStreamNnz = CFDX_STREAM_NNZ;  // staged products per row block
 constexpr int kMaxRowsPerBlock = 1024;
 constexpr int kXcds = 8;
 

@@ -139,13 +139,16 @@ __device__ __forceinline__ f32x4 mfma_tile(const uint16_t* A, int lda, const uin
     return acc;
 }
 
-__device__ __forceinline__ float inv_rho_of(const double* scal, uint32_t n) {
-    const double rho = std::sqrt(scal[0] / double(n));
-    return rho > 0.0 ? float(1.0 / rho) : 0.0f;
+// rho and 1/rho, computed once per forward pass into scal[3], scal[4].
+__global__ void rho_kernel(double* scal, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double rho = std::sqrt(scal[0] / double(n));
+        scal[3] = rho;
+        scal[4] = rho > 0.0 ? 1.0 / rho : 0.0;
+    }
 }
-__device__ __forceinline__ float rho_of(const double* scal, uint32_t n) {
-    return float(std::sqrt(scal[0] / double(n)));
-}
+__device__ __forceinline__ float inv_rho_of(const double* scal, uint32_t) { return float(scal[4]); }
+__device__ __forceinline__ float rho_of(const double* scal, uint32_t) { return float(scal[3]); }
 
 // s_i = sqrt(inv_diag_i); partial[block] = sum (r_i s_i)^2 (grid stride).
 __global__ __launch_bounds__(kBlock) void scale_kernel(uint32_t n, const float* __restrict__ r,
@@ -594,6 +597,7 @@ void Mpnn::forward(DeviceAmg& amg, const float* r, float* e, bool keep) {
     const uint32_t rg = red_grid(n);
     scale_kernel<<<rg, kBlock, 0, st>>>(n, r, g.inv_diag, s_, part_);
     reduce_partials_kernel<<<1, 1024, 0, st>>>(part_, rg, kMaxPartBlocks, 1, scal_);
+    rho_kernel<<<1, 1, 0, st>>>(scal_, n);
     const uint32_t gn = uint32_t((uint64_t(n) * kHid + kBlock - 1) / kBlock);
     encode_kernel<<<gn, kBlock, 0, st>>>(n, r, s_, scal_, p_, h_[0], keep ? mask_[0] : nullptr);
     const uint32_t tiles = tiles_of(n);

@@ -68,11 +68,9 @@ public:
     uint32_t pairs_n = 0, pairs_count = 0, pairs_cap = 0;
     void reserve_pairs(uint32_t n, uint32_t count);
 
-    // Reference per-iteration rate (log10 residual reduction per FGMRES
-    // iteration) of the plain mixed method, and the fallback ratio.
-    double reference_rate = 0.0;
-    double fallback_ratio = 0.5;
-    uint32_t fallback_window = 5;
+    // Fallback: minimum per-iteration residual reduction over the window.
+    double fallback_min_reduction = 0.5;
+    uint32_t fallback_window = 3;
 
 private:
     void ensure(uint32_t n);
@@ -90,7 +88,7 @@ private:
     uint16_t* w_ = nullptr;   // per layer W_l as BF16, [64][32]
     // workspace
     float* s_ = nullptr;        // 1/sqrt(a_ii)
-    double* scal_ = nullptr;    // device scalars: 0 sum (r s)^2, 1 e.r, 2 v.Av
+    double* scal_ = nullptr;    // device scalars: 0 sum (r s)^2, 1 e.r, 2 v.Av, 3 rho, 4 1/rho
     double* part_ = nullptr;    // reduction partials
     uint16_t* h_[kMpnnLayers + 1] = {};  // BF16 hidden states
     uint32_t* mask_[kMpnnLayers + 1] = {};

@@ -169,6 +169,12 @@ cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* host_seco
  * of the last batch (HIP events on the handle's stream) is returned. */
 cfdxhip_status cfdxhip_amg_spmv_device(cfdxhip_amg* h, uint32_t precision, const void* x, void* y,
                                        uint32_t repeat, double* seconds_per_launch);
+/* Measurement: seconds per launch of one hierarchy level's kernel on the
+ * level's own buffers, timed with HIP events on the handle's stream.
+ * kind 0: y = A x; kind 1: the fused damped-Jacobi sweep. precision 0 FP64,
+ * 1 FP32. No reference counterpart. */
+cfdxhip_status cfdxhip_amg_level_kernel_seconds(cfdxhip_amg* h, size_t level, uint32_t precision, uint32_t kind,
+                                                uint32_t repeat, double* seconds);
 
 /* ------------------------------------------------------------ meshes ---- */
 
@@ -294,11 +300,13 @@ cfdxhip_status cfdxhip_mpnn_collect_pairs(cfdxhip_mpnn* m, cfdxhip_amg* h, const
  * 1/2 |(I - V A)(e - N(r))|_A^2 / (e^T r). losses[steps] (optional) receives
  * each step's loss. */
 cfdxhip_status cfdxhip_mpnn_train(cfdxhip_mpnn* m, cfdxhip_amg* h, uint32_t steps, float lr, double* losses);
-/* Fallback rule: inside a learned solve the stage is dropped (plain V-cycle,
- * counted in learned_stage_fallbacks) once the residual reduction per
- * iteration, log10(|g_{k-window}| / |g_k|) / window, falls below
- * ratio * reference_rate; reference_rate is the plain method's rate. */
-cfdxhip_status cfdxhip_mpnn_set_fallback(cfdxhip_mpnn* m, double reference_rate, double ratio, uint32_t window);
+/* Fallback rule: inside a learned solve the stage is dropped for the rest of
+ * the solve (plain V-cycle, counted in learned_stage_fallbacks) once the
+ * residual reduction per iteration over the last `window` iterations,
+ * 1 - (|g_k| / |g_{k-window}|)^(1/window) from the Givens estimates, drops
+ * below min_reduction (default 0.5, window 3; min_reduction <= 1, a negative
+ * value keeps the stage on). */
+cfdxhip_status cfdxhip_mpnn_set_fallback(cfdxhip_mpnn* m, double min_reduction, uint32_t window);
 /* Mixed FGMRES with the learned stage ahead of the FP32 V-cycle (host or
  * device vectors). Same contract as cfdxhip_amg_solve_fgmres. */
 cfdxhip_status cfdxhip_amg_solve_fgmres_learned(cfdxhip_amg* h, cfdxhip_mpnn* m, const double* b, double* x,
