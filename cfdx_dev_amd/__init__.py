@@ -16,6 +16,7 @@ All numerical work runs in libcfdxhip.so on the GPU; nothing here computes.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -472,3 +473,67 @@ class LearnedStage:
         """Drop the stage once the per-iteration residual reduction over
         `window` iterations falls below min_reduction (negative: never)."""
         self._check(self._lib.cfdxhip_mpnn_set_fallback(self._h, min_reduction, window))
+
+
+# ----------------------------------------------------- capture files --
+
+PROBLEMS = {"general": 0, "pressure_poisson": 1, "diffusion": 2, "momentum": 3, "scalar_transport": 4,
+            "coupled_pressure_velocity": 5}
+
+
+@dataclass
+class LinearSystemCapture:
+    """One captured system (core/linalg/linear_system_capture.h:73-83)."""
+    rowp: np.ndarray
+    cols: np.ndarray
+    vals: np.ndarray
+    rhs: np.ndarray
+    initial_solution: np.ndarray
+    problem: int = 0
+    null_space: int = 0
+    scaling: int = 0
+    max_iterations: int = 1000
+    tolerance: float = 1e-12
+    checksum: int = 0
+
+    @property
+    def n(self) -> int:
+        return len(self.rowp) - 1
+
+
+def read_capture(path) -> LinearSystemCapture:
+    """Reads a .cfdxlin file (CFDX_LINEAR_REPLAY_V1); raises ValueError with
+    the reader's reason when the file is rejected."""
+    lib = _native.load()
+    h = C.c_void_p()
+    if lib.cfdxhip_capture_read(os.fsencode(path), C.byref(h)) != 0:
+        raise ValueError((lib.cfdxhip_capture_last_error() or b"").decode())
+    try:
+        info = _native.CaptureInfo()
+        lib.cfdxhip_capture_info_get(h, C.byref(info))
+        n, nnz = info.n, info.nnz
+        rp, cols = np.empty(n + 1, np.uint32), np.empty(nnz, np.uint32)
+        vals, rhs, x0 = np.empty(nnz), np.empty(n), np.empty(n)
+        lib.cfdxhip_capture_copy(h, rp, cols, vals, rhs, x0)
+        return LinearSystemCapture(rp, cols, vals, rhs, x0, info.problem, info.null_space, info.scaling,
+                                   info.max_iterations, info.tolerance, info.checksum)
+    finally:
+        lib.cfdxhip_capture_destroy(h)
+
+
+def write_capture(path, cap: LinearSystemCapture) -> int:
+    """Writes a .cfdxlin file; returns its checksum. Raises ValueError when
+    the system fails the format's validation."""
+    lib = _native.load()
+    info = _native.CaptureInfo()
+    info.n = cap.n
+    info.nnz = len(cap.vals)
+    info.max_iterations = cap.max_iterations
+    info.problem, info.null_space, info.scaling = cap.problem, cap.null_space, cap.scaling
+    info.tolerance = cap.tolerance
+    out = C.c_uint64()
+    st = lib.cfdxhip_capture_write(os.fsencode(path), C.byref(info), _u32(cap.rowp), _u32(cap.cols), _f64(cap.vals),
+                                   _f64(cap.rhs), _f64(cap.initial_solution), C.byref(out))
+    if st != 0:
+        raise ValueError((lib.cfdxhip_capture_last_error() or b"").decode())
+    return out.value

@@ -98,6 +98,20 @@ class MeshSizes(C.Structure):
     ]
 
 
+class CaptureInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("nnz", C.c_uint64),
+        ("max_iterations", C.c_uint64),
+        ("checksum", C.c_uint64),
+        ("problem", C.c_uint32),
+        ("null_space", C.c_uint32),
+        ("scaling", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("tolerance", C.c_double),
+    ]
+
+
 def _sig(lib, name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -136,6 +150,13 @@ def load():
     _sig(lib, "cfdxhip_amg_smoother_omega", st, [vp, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_level_lambda_max", st, [vp, C.c_size_t, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_set_chebyshev", st, [vp, C.c_uint32, C.c_double])
+    _sig(lib, "cfdxhip_capture_read", st, [C.c_char_p, C.POINTER(vp)])
+    _sig(lib, "cfdxhip_capture_info_get", st, [vp, C.POINTER(CaptureInfo)])
+    _sig(lib, "cfdxhip_capture_copy", st, [vp, u32p, u32p, f64p, f64p, f64p])
+    _sig(lib, "cfdxhip_capture_destroy", None, [vp])
+    _sig(lib, "cfdxhip_capture_write", st,
+         [C.c_char_p, C.POINTER(CaptureInfo), u32p, u32p, f64p, f64p, f64p, C.POINTER(C.c_uint64)])
+    _sig(lib, "cfdxhip_capture_last_error", C.c_char_p, [])
     _sig(lib, "cfdxhip_amg_level_kernel_seconds", st,
          [vp, C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_mpnn_param_count", C.c_size_t, [])

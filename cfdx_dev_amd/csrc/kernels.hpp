@@ -129,21 +129,29 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
         // All of this thread's matrix loads are issued before the dependent
         // gathers of x; values and columns are read once, so they bypass the
         // caches (non-temporal) and leave L2/MALL to x.
+        // The gathers are unconditional (out-of-range slots read x[0]) so
+        // that all of them are in flight together; a predicated gather
+        // would wait for each load before issuing the next.
         constexpr int kPer = kStreamNnz / kBlock;
         uint32_t ci[kPer];
         T vi[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const uint32_t k = k0 + threadIdx.x + uint32_t(j) * kBlock;
+            ci[j] = 0;
+            vi[j] = T(0);
             if (k < k1) {
                 ci[j] = __builtin_nontemporal_load(cols + k);
                 vi[j] = __builtin_nontemporal_load(vals + k);
             }
         }
+        T xv[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) xv[j] = x[ci[j]];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const uint32_t k = k0 + threadIdx.x + uint32_t(j) * kBlock;
-            if (k < k1) prod[k - k0] = vi[j] * x[ci[j]];
+            if (k < k1) prod[k - k0] = vi[j] * xv[j];
         }
         __syncthreads();
         if (kOrdered || (k1 - k0) < kGroupRowNnz * (r1 - r0)) {

@@ -315,6 +315,44 @@ cfdxhip_status cfdxhip_amg_solve_fgmres_learned_device(cfdxhip_amg* h, cfdxhip_m
                                                        double* x, int restart, size_t max_iter, double tol,
                                                        cfdxhip_result* result);
 
+/* ------------------------------------------------------------------------
+ * Linear-system capture files (.cfdxlin, CFDX_LINEAR_REPLAY_V1): read and
+ * write the reference's replay format (core/linalg/linear_system_capture.h:
+ * 165-284: write_linear_system_capture, read_linear_system_capture), byte for
+ * byte, with its FNV-1a checksum and limits (rows <= 2,000,000,
+ * nnz <= 100,000,000). The reference's file-level validation applies: square
+ * finalized CSR with strictly ascending columns, finite values, known enum
+ * values, 0 < max_iterations <= 1e8, finite tolerance > 0.
+ * ------------------------------------------------------------------------ */
+typedef struct cfdxhip_capture cfdxhip_capture;
+typedef struct cfdxhip_capture_info {
+    uint64_t n;               /* rows = columns */
+    uint64_t nnz;
+    uint64_t max_iterations;
+    uint64_t checksum;        /* read: the file's; write: ignored (computed) */
+    uint32_t problem;         /* LinearProblemKind 0..5 (linear_solver_models.h:11-18) */
+    uint32_t null_space;      /* NullSpaceModel 0..1 (:59) */
+    uint32_t scaling;         /* LinearScalingModel 0..4 (:61) */
+    uint32_t reserved;
+    double tolerance;
+} cfdxhip_capture_info;
+
+/* Parses and verifies a capture (checksum, CSR, limits). On failure the
+ * reason is in cfdxhip_capture_last_error() (per thread). */
+cfdxhip_status cfdxhip_capture_read(const char* path, cfdxhip_capture** out);
+cfdxhip_status cfdxhip_capture_info_get(const cfdxhip_capture* capture, cfdxhip_capture_info* info);
+/* Copies the system out: row_offsets[n+1], columns[nnz], values[nnz],
+ * rhs[n], initial_solution[n]. */
+cfdxhip_status cfdxhip_capture_copy(const cfdxhip_capture* capture, uint32_t* row_offsets, uint32_t* columns,
+                                    double* values, double* rhs, double* initial_solution);
+void cfdxhip_capture_destroy(cfdxhip_capture* capture);
+/* Writes a capture atomically (temporary file + rename); checksum_out
+ * (optional) receives the checksum written. */
+cfdxhip_status cfdxhip_capture_write(const char* path, const cfdxhip_capture_info* info, const uint32_t* row_offsets,
+                                     const uint32_t* columns, const double* values, const double* rhs,
+                                     const double* initial_solution, uint64_t* checksum_out);
+const char* cfdxhip_capture_last_error(void);
+
 /* Version string of the library build. */
 const char* cfdxhip_version(void);
 

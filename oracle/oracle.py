@@ -293,3 +293,36 @@ class OracleAmg:
         res = OracleResult()
         self._fn("fgmres")(self.h, _f(b, np.float64), x, restart, max_iter, tol, C.byref(res))
         return res
+
+
+def ref_capture_write(path, rowp, cols, vals, rhs, x0, problem=0, null_space=0, scaling=0, max_iterations=1000,
+                      tolerance=1e-12):
+    """Writes a capture with the reference's own writer (oracle/_ref)."""
+    o = Oracle(reference=True)
+    f = o.lib.ref_capture_write
+    f.restype = C.c_int
+    arrs = [_f(rowp, np.uint32), _f(cols, np.uint32), _f(vals, np.float64), _f(rhs, np.float64), _f(x0, np.float64)]
+    ptrs = [a.ctypes.data_as(C.c_void_p) for a in arrs]
+    return f(os.fsencode(path), C.c_size_t(len(rowp) - 1), *ptrs, problem, null_space, scaling,
+             C.c_size_t(max_iterations), C.c_double(tolerance)) == 0
+
+
+def ref_capture_read(path):
+    """Reads a capture with the reference's own reader; None when rejected."""
+    o = Oracle(reference=True)
+    f = o.lib.ref_capture_read
+    f.restype = C.c_int
+    sizes = (C.c_size_t * 2)()
+    nul = C.c_void_p()
+    meta = (C.c_int * 3)()
+    mi, tol, chk = C.c_size_t(), C.c_double(), C.c_uint64()
+    if f(os.fsencode(path), sizes, nul, nul, nul, nul, nul, meta, C.byref(mi), C.byref(tol), C.byref(chk)) != 0:
+        return None
+    n, nnz = sizes[0], sizes[1]
+    rp, cols = np.empty(n + 1, np.uint32), np.empty(nnz, np.uint32)
+    vals, rhs, x0 = np.empty(nnz), np.empty(n), np.empty(n)
+    vp = C.c_void_p
+    f(os.fsencode(path), sizes, rp.ctypes.data_as(vp), cols.ctypes.data_as(vp), vals.ctypes.data_as(vp),
+      rhs.ctypes.data_as(vp), x0.ctypes.data_as(vp), meta, C.byref(mi), C.byref(tol), C.byref(chk))
+    return dict(rowp=rp, cols=cols, vals=vals, rhs=rhs, x0=x0, problem=meta[0], null_space=meta[1],
+                scaling=meta[2], max_iterations=mi.value, tolerance=tol.value, checksum=chk.value)

@@ -52,6 +52,7 @@
 #include "cfdx/core/linalg/hypre_amg.h"
 #include "cfdx/core/linalg/cg_solver.h"
 #include "cfdx/core/linalg/gmres_solver.h"
+#include "cfdx/core/linalg/linear_system_capture.h"
 #include "cfdx/core/solvers/scalar_diffusion.h"
 #include "cfdx/physics/steady_incompressible_solver.h"
 
@@ -560,6 +561,58 @@ int ref_fgmres(void* hp, const double* b, double* x, int restart, size_t max_ite
     res->residual_relative = r.residual_relative;
     res->residual_replacements = r.residual_replacements;
     return 0;
+}
+
+
+// ---- linear-system capture (core/linalg/linear_system_capture.h:165-284) ----
+// Writes with the reference's writer; returns 0 on success.
+int ref_capture_write(const char* path, size_t n, const uint32_t* rowp, const uint32_t* cols, const double* vals,
+                      const double* rhs, const double* x0, int problem, int null_space, int scaling,
+                      size_t max_iterations, double tolerance) {
+    try {
+        const SparseMatrix A = csr_from(n, rowp, cols, vals);
+        Vector b(n, 0.0), x(n, 0.0);
+        for (size_t i = 0; i < n; ++i) {
+            b(i) = rhs[i];
+            x(i) = x0[i];
+        }
+        write_linear_system_capture(path, A, b, x, static_cast<LinearProblemKind>(problem),
+                                    static_cast<NullSpaceModel>(null_space),
+                                    static_cast<LinearScalingModel>(scaling), max_iterations, tolerance);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+// Reads with the reference's reader into caller buffers sized from sizes[]
+// (n, nnz) of a first call with null buffers; returns 0 on success, 1 when
+// the reference rejects the file.
+int ref_capture_read(const char* path, size_t* sizes, uint32_t* rowp, uint32_t* cols, double* vals, double* rhs,
+                     double* x0, int* meta, size_t* max_iterations, double* tolerance, uint64_t* checksum) {
+    try {
+        const LinearSystemCapture c = read_linear_system_capture(path);
+        const size_t n = c.matrix.n_rows(), nnz = c.matrix.nnz();
+        sizes[0] = n;
+        sizes[1] = nnz;
+        if (!rowp) return 0;
+        std::memcpy(rowp, c.matrix.row_offsets_data(), (n + 1) * 4);
+        std::memcpy(cols, c.matrix.columns_data(), nnz * 4);
+        std::memcpy(vals, c.matrix.values_data(), nnz * 8);
+        for (size_t i = 0; i < n; ++i) {
+            rhs[i] = c.rhs(i);
+            x0[i] = c.initial_solution(i);
+        }
+        meta[0] = static_cast<int>(c.problem);
+        meta[1] = static_cast<int>(c.null_space);
+        meta[2] = static_cast<int>(c.scaling);
+        *max_iterations = c.max_iterations;
+        *tolerance = c.tolerance;
+        *checksum = c.checksum;
+        return 0;
+    } catch (...) {
+        return 1;
+    }
 }
 
 }  // extern "C"
