@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 # 2.0, the factor that makes it read the known size of a 1 GiB device copy.
 MEASURED_TRAFFIC = {
     ("box256_walls30_ar1000_lognormal1.5", "fp32"): {
-        "bytes": 1170409681,
+        "bytes": 1167249711,
         "source": "profiles/spmv_fp32_box256_hbm_traffic.md",
     },
 }
@@ -269,6 +269,20 @@ def main():
         hip.hipFree(db)
         return 2 * nbytes / dt / 1e9
 
+    # Per-level kernel rates (HIP events, level buffers): SpMV and fused
+    # Jacobi sweep, FP32 and FP64, algorithmic bytes as in spmv_bytes.
+    level_rates = []
+    for l in range(min(4, h.num_levels())):
+        nl, nnzl, _ = h.level_info(l)
+        rec = {"level": l, "rows": int(nl), "nnz": int(nnzl)}
+        for prec, sz in (("fp32", 4), ("fp64", 8)):
+            for kind, extra in (("spmv", 0), ("jacobi", 3)):
+                t = h.level_kernel_seconds(l, prec, kind, 10)
+                by = spmv_bytes(nl, nnzl, sz, sz) + extra * nl * sz
+                rec[f"{prec}_{kind}"] = {"us": round(t * 1e6, 1), "TBps": round(by / t / 1e12, 3)}
+        level_rates.append(rec)
+    log(f"[bench] per-level kernels: {level_rates}")
+
     copy_gbs = copy_rate(1 << 30)
     log(f"[bench] device copy 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
     t64 = spmv_time(plain, 0, np.float64)
@@ -344,6 +358,7 @@ def main():
         "spmv_fp32_TBps": round(gbs32 / 1e3, 4),
         "spmv_fp64_frac_of_peak": round(gbs64 / HBM_PEAK_GBS, 4),
         "device_copy_TBps": round(copy_gbs / 1e3, 4),
+        "level_kernels": level_rates,
         "roofline": {
             "bound": "hbm",
             "achieved": round(gbs32, 1),

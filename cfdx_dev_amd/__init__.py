@@ -204,6 +204,12 @@ class NativeAMGPreconditioner:
             raise RuntimeError(self.last_error())
         return v.value
 
+    def counters(self):
+        """(coarse_size, hierarchy_builds, numeric_updates), hypre_amg.h:72,86-87."""
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._lib.cfdxhip_amg_counters(self._h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
     def set_chebyshev(self, degree: int, ratio: float = 30.0) -> bool:
         self.last_status = self._lib.cfdxhip_amg_set_chebyshev(self._h, degree, ratio)
         return self.last_status == 0

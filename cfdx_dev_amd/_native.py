@@ -150,6 +150,8 @@ def load():
     _sig(lib, "cfdxhip_amg_smoother_omega", st, [vp, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_level_lambda_max", st, [vp, C.c_size_t, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_set_chebyshev", st, [vp, C.c_uint32, C.c_double])
+    _sig(lib, "cfdxhip_amg_counters", st,
+         [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
     _sig(lib, "cfdxhip_capture_read", st, [C.c_char_p, C.POINTER(vp)])
     _sig(lib, "cfdxhip_capture_info_get", st, [vp, C.POINTER(CaptureInfo)])
     _sig(lib, "cfdxhip_capture_copy", st, [vp, u32p, u32p, f64p, f64p, f64p])

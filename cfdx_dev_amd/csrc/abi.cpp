@@ -33,6 +33,7 @@ struct cfdxhip_amg {
     hipStream_t stream = nullptr;
     std::string error;
     double host_setup_s = 0.0, device_setup_s = 0.0;
+    uint64_t hierarchy_builds = 0, numeric_updates = 0;
     bool ready = false;
 };
 
@@ -261,6 +262,7 @@ cfdxhip_status cfdxhip_amg_setup(cfdxhip_amg* h, size_t n, const uint32_t* rowp,
         h->host_setup_s = std::chrono::duration<double>(t1 - t0).count();
         h->device_setup_s = std::chrono::duration<double>(t2 - t1).count();
         h->ready = true;
+        ++h->hierarchy_builds;
         h->error.clear();
         return CFDXHIP_OK;
     });
@@ -281,6 +283,7 @@ cfdxhip_status cfdxhip_amg_update_values(cfdxhip_amg* h, const double* vals) {
         }
         if (h->dev && !h->dev->refresh(staged, err)) return fail(h, CFDXHIP_ERR_DEVICE, err);
         h->host = std::move(staged);
+        ++h->numeric_updates;
         h->error.clear();
         return CFDXHIP_OK;
     });
@@ -442,6 +445,15 @@ cfdxhip_status cfdxhip_amg_set_chebyshev(cfdxhip_amg* h, uint32_t degree, double
     h->cfg.cheb_degree = degree;
     h->cfg.cheb_ratio = ratio;
     if (h->dev) h->dev->set_chebyshev_params(h->host, degree, ratio);
+    return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_counters(const cfdxhip_amg* h, uint64_t* coarse_size, uint64_t* hierarchy_builds,
+                                    uint64_t* numeric_updates) {
+    if (!h || !coarse_size || !hierarchy_builds || !numeric_updates) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    *coarse_size = h->ready && !h->host.lv.empty() ? h->host.lv.back().A.n : 0;
+    *hierarchy_builds = h->hierarchy_builds;
+    *numeric_updates = h->numeric_updates;
     return CFDXHIP_OK;
 }
 

@@ -152,6 +152,10 @@ cfdxhip_status cfdxhip_amg_level_matrix(const cfdxhip_amg* h, size_t level, uint
 cfdxhip_status cfdxhip_amg_level_transfer(const cfdxhip_amg* h, size_t level, uint64_t* aggregate,
                                           uint32_t* p_row_offsets, uint32_t* p_columns, double* p_values);
 cfdxhip_status cfdxhip_amg_smoother_omega(const cfdxhip_amg* h, double* omega);
+/* coarse_size(), hierarchy_builds(), numeric_updates() (hypre_amg.h:72,86-87):
+ * size of the coarsest level, successful setups, successful numeric updates. */
+cfdxhip_status cfdxhip_amg_counters(const cfdxhip_amg* h, uint64_t* coarse_size, uint64_t* hierarchy_builds,
+                                    uint64_t* numeric_updates);
 /* Lanczos estimate of lambda_max(D^-1 A) on a level, the Chebyshev
  * smoother's interval anchor. CFDXHIP_ERR_NOT_READY unless a Chebyshev
  * smoother is configured. No reference counterpart. */
