@@ -126,6 +126,19 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
     const uint32_t k0 = rowp[r0], k1 = rowp[r1];
     double part = 0.0;
     if (k1 - k0 <= uint32_t(kStreamNnz)) {
+        // The epilogue inputs of this thread's first row (its offsets, the
+        // initial value and the epilogue's vectors) are loaded before the
+        // matrix stream, so their latency hides behind it.
+        const uint32_t row1 = r0 + threadIdx.x;
+        typename Epi::Pre pre{};
+        T ini = T(0);
+        uint32_t rb = 0, re = 0;
+        if (row1 < r1) {
+            rb = rowp[row1];
+            re = rowp[row1 + 1];
+            pre = epi.load(row1);
+            ini = init(row1);
+        }
         // All of this thread's matrix loads are issued before the dependent
         // gathers of x; values and columns are read once, so they bypass the
         // caches (non-temporal) and leave L2/MALL to x.
@@ -154,23 +167,44 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
             if (k < k1) prod[k - k0] = vi[j] * xv[j];
         }
         __syncthreads();
-        if (kOrdered || (k1 - k0) < kGroupRowNnz * (r1 - r0)) {
-            for (uint32_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
-                const T s = ordered_sum(prod, rowp[row] - k0, rowp[row + 1] - k0, init(row));
-                part += epi(row, s);
+        bool grouped = false;
+        if constexpr (!kOrdered) grouped = (k1 - k0) >= kGroupRowNnz * (r1 - r0);
+        if (!grouped) {
+            for (uint32_t row = row1; row < r1; row += kBlock) {
+                if (row != row1) {
+                    rb = rowp[row];
+                    re = rowp[row + 1];
+                    pre = epi.load(row);
+                    ini = init(row);
+                }
+                const T s = ordered_sum(prod, rb - k0, re - k0, ini);
+                part += epi.apply(row, s, pre);
             }
         } else {
-            // Long rows (FP32 only, no ordering contract): 16 lanes per row,
-            // strided partial sums combined by a shuffle tree.
-            const uint32_t g = threadIdx.x >> 4, lane = threadIdx.x & 15;
-            for (uint32_t row0 = r0; row0 < r1; row0 += kBlock / 16) {
-                const uint32_t row = row0 + g;
-                T s = T(0);
-                if (row < r1)
-                    for (uint32_t k = rowp[row] - k0 + lane; k < rowp[row + 1] - k0; k += 16) s += prod[k];
+            if constexpr (!kOrdered) {
+                // Long rows (FP32 only, no ordering contract): 16 lanes per
+                // row, strided partial sums combined by a shuffle tree, row
+                // sums staged in LDS; then one thread per row applies the
+                // epilogue, so the epilogue's loads are issued once, together.
+                __shared__ T rsum[kMaxRowsPerBlock];
+                const uint32_t g = threadIdx.x >> 4, lane = threadIdx.x & 15;
+                for (uint32_t row0 = r0; row0 < r1; row0 += kBlock / 16) {
+                    const uint32_t row = row0 + g;
+                    T s = T(0);
+                    if (row < r1)
+                        for (uint32_t k = rowp[row] - k0 + lane; k < rowp[row + 1] - k0; k += 16) s += prod[k];
 #pragma unroll
-                for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
-                if (row < r1 && lane == 0) part += epi(row, init(row) + s);
+                    for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+                    if (row < r1 && lane == 0) rsum[row - r0] = s;
+                }
+                __syncthreads();
+                for (uint32_t row = row1; row < r1; row += kBlock) {
+                    if (row != row1) {
+                        pre = epi.load(row);
+                        ini = init(row);
+                    }
+                    part += epi.apply(row, ini + rsum[row - r0], pre);
+                }
             }
         }
     } else if (kOrdered) {
@@ -212,15 +246,22 @@ struct InitFrom {
     __device__ T operator()(uint32_t row) const { return v[row]; }
 };
 
+// Epilogue functors: load(row) fetches the row's inputs (issued early by
+// the kernel), apply(row, s, pre) finishes the row from its sum s.
+struct NoPre {};
+
 // y = A x
 template <class T>
 struct EpiStore {
     static constexpr bool kPartial = false;
+    using Pre = NoPre;
     T* y;
-    __device__ double operator()(uint32_t row, T s) const {
+    __device__ Pre load(uint32_t) const { return {}; }
+    __device__ double apply(uint32_t row, T s, const Pre&) const {
         y[row] = s;
         return 0.0;
     }
+    __device__ double operator()(uint32_t row, T s) const { return apply(row, s, load(row)); }
     __device__ void store_partial(uint32_t, double) const {}
 };
 
@@ -228,13 +269,18 @@ struct EpiStore {
 template <class T>
 struct EpiStoreDot {
     static constexpr bool kPartial = true;
+    struct Pre {
+        T w;
+    };
     T* y;
     const T* w;
     double* partial;
-    __device__ double operator()(uint32_t row, T s) const {
+    __device__ Pre load(uint32_t row) const { return {w[row]}; }
+    __device__ double apply(uint32_t row, T s, const Pre& p) const {
         y[row] = s;
-        return double(w[row]) * double(s);
+        return double(p.w) * double(s);
     }
+    __device__ double operator()(uint32_t row, T s) const { return apply(row, s, load(row)); }
     __device__ void store_partial(uint32_t b, double v) const { partial[b] = v; }
 };
 
@@ -242,15 +288,20 @@ struct EpiStoreDot {
 template <class T, bool kNorm>
 struct EpiResidual {
     static constexpr bool kPartial = kNorm;
+    struct Pre {
+        T b;
+    };
     const T* b;
     T* r;
     double* partial;
-    __device__ double operator()(uint32_t row, T s) const {
+    __device__ Pre load(uint32_t row) const { return {b[row]}; }
+    __device__ double apply(uint32_t row, T s, const Pre& p) const {
 #pragma clang fp contract(off)
-        const T v = b[row] - s;
+        const T v = p.b - s;
         r[row] = v;
         return kNorm ? double(v) * double(v) : 0.0;
     }
+    __device__ double operator()(uint32_t row, T s) const { return apply(row, s, load(row)); }
     __device__ void store_partial(uint32_t bb, double v) const { partial[bb] = v; }
 };
 
@@ -259,17 +310,22 @@ struct EpiResidual {
 template <class T>
 struct EpiJacobi {
     static constexpr bool kPartial = false;
+    struct Pre {
+        T b, x, inv;
+    };
     const T* b;
     const T* x;
     const T* inv_diag;
     T* xn;
     T omega;
-    __device__ double operator()(uint32_t row, T s) const {
+    __device__ Pre load(uint32_t row) const { return {b[row], x[row], inv_diag[row]}; }
+    __device__ double apply(uint32_t row, T s, const Pre& p) const {
 #pragma clang fp contract(off)
-        const T delta = inv_diag[row] * (b[row] - s);
-        xn[row] = x[row] + omega * delta;
+        const T delta = p.inv * (p.b - s);
+        xn[row] = p.x + omega * delta;
         return 0.0;
     }
+    __device__ double operator()(uint32_t row, T s) const { return apply(row, s, load(row)); }
     __device__ void store_partial(uint32_t, double) const {}
 };
 
@@ -279,20 +335,25 @@ struct EpiJacobi {
 template <class T, bool kFirst>
 struct EpiCheb {
     static constexpr bool kPartial = false;
+    struct Pre {
+        T b, x, inv, d;
+    };
     const T* b;
     const T* x;
     const T* inv_diag;
     T* d;
     T* xn;
     T c1, c2;
-    __device__ double operator()(uint32_t row, T s) const {
+    __device__ Pre load(uint32_t row) const { return {b[row], x[row], inv_diag[row], kFirst ? T(0) : d[row]}; }
+    __device__ double apply(uint32_t row, T s, const Pre& p) const {
 #pragma clang fp contract(off)
-        const T delta = inv_diag[row] * (b[row] - s);
-        const T dn = kFirst ? c2 * delta : c1 * d[row] + c2 * delta;
+        const T delta = p.inv * (p.b - s);
+        const T dn = kFirst ? c2 * delta : c1 * p.d + c2 * delta;
         d[row] = dn;
-        xn[row] = x[row] + dn;
+        xn[row] = p.x + dn;
         return 0.0;
     }
+    __device__ double operator()(uint32_t row, T s) const { return apply(row, s, load(row)); }
     __device__ void store_partial(uint32_t, double) const {}
 };
 
@@ -301,11 +362,14 @@ struct EpiCheb {
 template <class T>
 struct EpiAssign {
     static constexpr bool kPartial = false;
+    using Pre = NoPre;
     T* y;
-    __device__ double operator()(uint32_t row, T s) const {
+    __device__ Pre load(uint32_t) const { return {}; }
+    __device__ double apply(uint32_t row, T s, const Pre&) const {
         y[row] = s;
         return 0.0;
     }
+    __device__ double operator()(uint32_t row, T s) const { return apply(row, s, load(row)); }
     __device__ void store_partial(uint32_t, double) const {}
 };
 
