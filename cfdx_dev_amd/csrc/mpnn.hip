@@ -62,12 +62,16 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 __device__ __forceinline__ uint32_t pack2(float a, float b) { return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16); }
 
-// Eight channels [8q, 8q + 8) of node j: BF16 hidden states or FP32 rows.
+// Eight channels [8q, 8q + 8) of node j: BF16 hidden states (kept packed,
+// four registers per row, until they are accumulated) or FP32 rows.
 struct LoadBf8 {
+    using Raw = uint4;
     const uint16_t* h;
     uint32_t q;
-    __device__ __forceinline__ void operator()(uint32_t j, float* v) const {
-        const uint4 u = *reinterpret_cast<const uint4*>(h + size_t(j) * kHid + 8 * q);
+    __device__ __forceinline__ Raw raw(uint32_t j) const {
+        return *reinterpret_cast<const uint4*>(h + size_t(j) * kHid + 8 * q);
+    }
+    __device__ __forceinline__ static void unpack(const Raw& u, float* v) {
         const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -75,15 +79,23 @@ struct LoadBf8 {
             v[2 * t + 1] = __uint_as_float(w[t] & 0xffff0000u);
         }
     }
+    __device__ __forceinline__ void operator()(uint32_t j, float* v) const { unpack(raw(j), v); }
 };
 struct LoadF8 {
+    struct Raw {
+        float4 a, b;
+    };
     const float* g;
     uint32_t q;
-    __device__ __forceinline__ void operator()(uint32_t j, float* v) const {
-        const float4 a = *reinterpret_cast<const float4*>(g + size_t(j) * kHid + 8 * q);
-        const float4 b = *reinterpret_cast<const float4*>(g + size_t(j) * kHid + 8 * q + 4);
-        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+    __device__ __forceinline__ Raw raw(uint32_t j) const {
+        return {*reinterpret_cast<const float4*>(g + size_t(j) * kHid + 8 * q),
+                *reinterpret_cast<const float4*>(g + size_t(j) * kHid + 8 * q + 4)};
     }
+    __device__ __forceinline__ static void unpack(const Raw& r, float* v) {
+        v[0] = r.a.x, v[1] = r.a.y, v[2] = r.a.z, v[3] = r.a.w;
+        v[4] = r.b.x, v[5] = r.b.y, v[6] = r.b.z, v[7] = r.b.w;
+    }
+    __device__ __forceinline__ void operator()(uint32_t j, float* v) const { unpack(raw(j), v); }
 };
 
 // m[c] = sum_{j != i} w_ij x_j[8q + c], w_ij = (-a_ij s_i) s_j, in entry
@@ -106,18 +118,21 @@ __device__ __forceinline__ void gather8(uint32_t i, const uint32_t* __restrict__
             jj[u] = ok ? cols[k + u] : i;
             ww[u] = ok ? vals[k + u] : 0.0f;
         }
-        float sj[8], hv[8][8];
+        float sj[8];
+        typename Load::Raw raw[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             sj[u] = s[jj[u]];
-            ld(jj[u], hv[u]);
+            raw[u] = ld.raw(jj[u]);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             if (jj[u] == i) continue;
             const float w = (-ww[u] * si) * sj[u];
+            float hv[8];
+            Load::unpack(raw[u], hv);
 #pragma unroll
-            for (int c = 0; c < 8; ++c) m[c] += w * hv[u][c];
+            for (int c = 0; c < 8; ++c) m[c] += w * hv[c];
         }
     }
 }
