@@ -311,8 +311,10 @@ def main():
             "training": {"boxes": f"{args.train_n}^3 x {args.train_systems} (seeds 1000..)",
                          "steps_per_system": args.train_steps, "pairs_per_system": len(train_meta["pair_iterations"]),
                          "seconds": round(t_train, 2),
-                         "loss_first": train_meta["history"][0]["loss_first"],
-                         "loss_last": train_meta["history"][-1]["loss_last"]},
+                         "systems_trained": train_meta["systems_trained"], "pairs": train_meta["pairs"],
+                         "loss_first": next(r["loss_first"] for r in train_meta["history"] if "loss_first" in r),
+                         "loss_last": next(r["loss_last"] for r in reversed(train_meta["history"])
+                                           if "loss_last" in r)},
         }
     out = {
         "metric": METRIC,

@@ -10,6 +10,7 @@ Adam steps on that system's graph. The plain mixed method's residual
 reduction per iteration on the same systems is recorded with the weights.
 
 Usage: python -m cfdx_dev_amd.train_mpnn [--n 48] [--systems 12] [--steps 120]
+                                         [--validate-n 64 --validate-systems 80]
                                          [--out cfdx_dev_amd/weights/mpnn_box.npy]
 Writes the float32 parameters (.npy) and a JSON sidecar (.json) with the
 reference rate and the training log.
@@ -54,17 +55,56 @@ def train(n=48, systems=12, steps=120, lr=2e-3, seed=0, device=0, tol=1e-8, cheb
         plain = h.solve_fgmres(s.b, x, 30, 1000, tol)
         if plain.status == 0 and plain.iterations > 0:
             rates.append(-np.log10(plain.residual_relative) / plain.iterations)
-        got, res = net.collect_pairs(h, s.b, PAIR_ITERATIONS, tol=1e-10)
+        try:
+            got, res = net.collect_pairs(h, s.b, PAIR_ITERATIONS, tol=1e-10)
+        except RuntimeError as e:
+            # The FP64 PCG can stop NOT_APPLICABLE on a system (as the
+            # reference's does); such a system yields no pairs.
+            rec = dict(system=k, seed=1000 + k, skipped=str(e))
+            history.append(rec)
+            log(json.dumps(rec))
+            continue
         losses = net.train(h, steps, lr)
         rec = dict(system=k, seed=1000 + k, pairs=got, pcg_iterations=int(res.iterations),
                    plain_fgmres_iterations=int(plain.iterations), loss_first=float(losses[:8].mean()),
                    loss_last=float(losses[-8:].mean()), seconds=round(time.perf_counter() - t0, 2))
         history.append(rec)
         log(json.dumps(rec))
-    meta = dict(n=n, systems=systems, steps=steps, lr=lr, seed=seed, pair_iterations=list(PAIR_ITERATIONS),
+    trained = [r for r in history if "skipped" not in r]
+    meta = dict(n=n, systems=systems, systems_trained=len(trained), pairs=int(sum(r["pairs"] for r in trained)),
+                steps=steps, lr=lr, seed=seed, pair_iterations=list(PAIR_ITERATIONS),
                 cheb_degree=cheb_degree, cheb_ratio=cheb_ratio,
                 plain_rate=float(np.mean(rates)) if rates else 0.0, history=history)
     return net.get_params(), meta
+
+
+def validate(params, n=64, systems=80, seed0=5000, device=0, tol=1e-8, cheb_degree=3, cheb_ratio=30.0,
+             log=print):
+    """Mixed FGMRES(30) + one FP32 V-cycle on unseen boxes (seeds seed0..),
+    without the stage and with it (fallback off): per-system iterations and
+    solve seconds, and their means."""
+    net = LearnedStage(device=device, params=params)
+    net.set_fallback(-1.0, 3)
+    rows = []
+    for k in range(systems):
+        s = W.c5_box(n=n, seed=seed0 + k, device=device)
+        h = amg_for(s, device, cheb_degree, cheb_ratio)
+        x = np.zeros(s.n)
+        plain = h.solve_fgmres(s.b, x, 30, 1000, tol)
+        x = np.zeros(s.n)
+        learned = h.solve_fgmres_learned(s.b, x, net, 30, 1000, tol)
+        rec = dict(seed=seed0 + k, plain_iterations=int(plain.iterations), learned_iterations=int(learned.iterations),
+                   plain_s=plain.solve_seconds, learned_s=learned.solve_seconds,
+                   ok=bool(plain.status == 0 and learned.status == 0))
+        rows.append(rec)
+        log(json.dumps(rec))
+    pi = np.array([r["plain_iterations"] for r in rows], float)
+    li = np.array([r["learned_iterations"] for r in rows], float)
+    return dict(n=n, systems=systems, all_converged=all(r["ok"] for r in rows),
+                mean_plain_iterations=float(pi.mean()), mean_learned_iterations=float(li.mean()),
+                systems_fewer_iterations=int((li < pi).sum()), systems_more_iterations=int((li > pi).sum()),
+                mean_plain_s=float(np.mean([r["plain_s"] for r in rows])),
+                mean_learned_s=float(np.mean([r["learned_s"] for r in rows])), rows=rows)
 
 
 def save(params, meta, out):
@@ -91,9 +131,18 @@ def main():
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cheb-degree", type=int, default=3)
+    ap.add_argument("--cheb-ratio", type=float, default=30.0)
+    ap.add_argument("--validate-n", type=int, default=0, help="validate on unseen boxes of this size (0: skip)")
+    ap.add_argument("--validate-systems", type=int, default=80)
     ap.add_argument("--out", default=DEFAULT_OUT)
     a = ap.parse_args()
-    params, meta = train(a.n, a.systems, a.steps, a.lr, a.seed)
+    params, meta = train(a.n, a.systems, a.steps, a.lr, a.seed, cheb_degree=a.cheb_degree, cheb_ratio=a.cheb_ratio)
+    if a.validate_n:
+        v = validate(params, a.validate_n, a.validate_systems, cheb_degree=a.cheb_degree, cheb_ratio=a.cheb_ratio)
+        meta["validation"] = {k: val for k, val in v.items() if k != "rows"}
+        meta["validation_rows"] = v["rows"]
+        print(json.dumps(meta["validation"]))
     save(params, meta, a.out)
     print(f"saved {a.out} (plain mixed method: {meta['plain_rate']:.4f} decades/iteration on the training boxes)")
 

@@ -131,6 +131,7 @@ def test_ablation_trained_stage_vs_plain_mixed():
     from cfdx_dev_amd.train_mpnn import amg_for, train
 
     params, meta = train(n=48, systems=12, steps=120, log=lambda m: None)
+    assert meta["systems_trained"] == 12
     assert meta["history"][-1]["loss_last"] < meta["history"][0]["loss_first"]
     s = W.c5_box(n=96, seed=5002)
     h = amg_for(s)
