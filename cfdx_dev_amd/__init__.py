@@ -375,6 +375,63 @@ class FvMesh:
                                                            reference_cell, vals, rhs))
         return rp, cols, vals, rhs
 
+    # --------------------------------------------------- momentum side --
+    def wls_gradient(self, bc_type, bc_value, field, condition_limit=1e8):
+        """compute_boundary_aware_wls_gradient (physics/finite_volume_transport.h:606-715):
+        (grad 3 x n, number of cells left at the Green-Gauss gradient)."""
+        g = np.empty(3 * self.mesh.n_cells)
+        fb = C.c_size_t()
+        self._check(self._lib.cfdxhip_fv_wls_gradient(self._h, np.ascontiguousarray(bc_type, np.uint8),
+                                                      _f64(bc_value), condition_limit, _f64(field), g,
+                                                      C.byref(fb)))
+        return g, fb.value
+
+    def momentum_pattern(self):
+        n, nnz = C.c_size_t(), C.c_size_t()
+        self._check(self._lib.cfdxhip_fv_momentum_pattern(self._h, C.byref(n), C.byref(nnz), None, None))
+        rp = np.empty(n.value + 1, np.uint32)
+        cols = np.empty(nnz.value, np.uint32)
+        self._check(self._lib.cfdxhip_fv_momentum_pattern(self._h, C.byref(n), C.byref(nnz), _ptr(rp), _ptr(cols)))
+        return rp, cols
+
+    def momentum_assemble(self, vbc_type, vbc_value, mu, phi, grad_p, body, U, bounded=True, alpha=1.0):
+        """The SIMPLE predictor's relaxed momentum equations
+        (steady_incompressible_solver.h:3262-3311, 3728-3770): one matrix for
+        the three components. Returns (rowp, cols, vals, rhs (3 x n), diag)."""
+        rp, cols = self.momentum_pattern()
+        n = self.mesh.n_cells
+        vals = np.empty(len(cols))
+        rhs = np.empty(3 * n)
+        diag = np.empty(n)
+        body_arr = None if body is None else _f64(body)
+        b = None if body_arr is None else _ptr(body_arr)
+        self._check(self._lib.cfdxhip_fv_momentum_assemble(self._h, np.ascontiguousarray(vbc_type, np.uint8),
+                                                           _f64(vbc_value), _f64(mu), _f64(phi), _f64(grad_p), b,
+                                                           _f64(U), int(bool(bounded)), alpha, vals, rhs, diag))
+        return rp, cols, vals, rhs, diag
+
+    def momentum_hbya(self, vals, rhs, diag, grad_p, U):
+        """rAU and HbyA (3 x n each): steady_incompressible_solver.h:7816-7836, 6430-6465."""
+        n = self.mesh.n_cells
+        rau = np.empty(3 * n)
+        hbya = np.empty(3 * n)
+        self._check(self._lib.cfdxhip_fv_momentum_hbya(self._h, _f64(vals), _f64(rhs), _f64(diag), _f64(grad_p),
+                                                       _f64(U), rau, hbya))
+        return rau, hbya
+
+    def velocity_correct(self, hbya, rau, grad_p):
+        U = np.empty(3 * self.mesh.n_cells)
+        self._check(self._lib.cfdxhip_fv_velocity_correct(self._h, _f64(hbya), _f64(rau), _f64(grad_p), U))
+        return U
+
+    def rhie_chow_flux(self, vbc_type, vbc_value, pbc_type, pbc_value, rho, hbya, p, rau):
+        """make_rhie_chow_mass_flux (steady_incompressible_solver.h:2063-2217)."""
+        phi = np.empty(self.mesh.n_faces)
+        self._check(self._lib.cfdxhip_fv_rhie_chow_flux(self._h, np.ascontiguousarray(vbc_type, np.uint8),
+                                                        _f64(vbc_value), np.ascontiguousarray(pbc_type, np.uint8),
+                                                        _f64(pbc_value), rho, _f64(hbya), _f64(p), _f64(rau), phi))
+        return phi
+
     def diffusion_assemble(self, gamma, source, face_value, face_is_neumann):
         nnz = C.c_size_t()
         self._check(self._lib.cfdxhip_fv_diffusion_assemble(self._h, gamma, None, None, None, C.byref(nnz), None,

@@ -197,6 +197,19 @@ def load():
                                                    f64p, f64p])
     _sig(lib, "cfdxhip_fv_pressure_assemble_device", st, [vp, u8p, f64p, C.c_double, vp, vp, vp, C.c_size_t,
                                                           vp, vp])
+    sz_p = C.POINTER(C.c_size_t)
+    _sig(lib, "cfdxhip_fv_wls_gradient", st, [vp, u8p, f64p, C.c_double, f64p, f64p, sz_p])
+    _sig(lib, "cfdxhip_fv_momentum_pattern", st, [vp, sz_p, sz_p, vp, vp])
+    _sig(lib, "cfdxhip_fv_momentum_assemble", st, [vp, u8p, f64p, f64p, f64p, f64p, vp, f64p, C.c_int32,
+                                                   C.c_double, f64p, f64p, f64p])
+    _sig(lib, "cfdxhip_fv_momentum_assemble_device", st, [vp, u8p, f64p, vp, vp, vp, vp, vp, C.c_int32,
+                                                          C.c_double, vp, vp, vp])
+    _sig(lib, "cfdxhip_fv_momentum_hbya", st, [vp, f64p, f64p, f64p, f64p, f64p, f64p, f64p])
+    _sig(lib, "cfdxhip_fv_momentum_hbya_device", st, [vp, vp, vp, vp, vp, vp, vp, vp])
+    _sig(lib, "cfdxhip_fv_velocity_correct", st, [vp, f64p, f64p, f64p, f64p])
+    _sig(lib, "cfdxhip_fv_velocity_correct_device", st, [vp, vp, vp, vp, vp])
+    _sig(lib, "cfdxhip_fv_rhie_chow_flux", st, [vp, u8p, f64p, u8p, f64p, C.c_double, f64p, f64p, f64p, f64p])
+    _sig(lib, "cfdxhip_fv_rhie_chow_flux_device", st, [vp, u8p, f64p, u8p, f64p, C.c_double, vp, vp, vp, vp])
     _sig(lib, "cfdxhip_fv_diffusion_assemble", st, [vp, C.c_double, vp, vp, vp, C.POINTER(C.c_size_t), vp, vp,
                                                     vp, vp])
     _lib = lib

@@ -725,6 +725,181 @@ cfdxhip_status cfdxhip_fv_diffusion_assemble(cfdxhip_fv* fv, double gamma, const
     });
 }
 
+// ------------------------------------------------------ momentum side --
+
+namespace {
+void h2d(void* d, const void* h, size_t bytes) { hip_check(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice), "copy"); }
+void d2h(void* h, const void* d, size_t bytes) { hip_check(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost), "copy"); }
+}  // namespace
+
+cfdxhip_status cfdxhip_fv_wls_gradient(cfdxhip_fv* fv, const uint8_t* bct, const double* bcv, double condition_limit,
+                                       const double* field, double* grad, size_t* fallback_cells) {
+    if (!fv || !field || !grad || (fv->mesh.npatch && (!bct || !bcv))) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        std::vector<uint8_t> t(std::max<uint32_t>(m.npatch, 1), 0);
+        std::vector<double> v(t.size(), 0.0);
+        if (m.npatch) {
+            std::memcpy(t.data(), bct, m.npatch);
+            std::memcpy(v.data(), bcv, size_t(m.npatch) * 8);
+        }
+        DevBuf df(size_t(m.nc) * 8), dg(size_t(m.nc) * 24), dfb(m.nc);
+        h2d(df.p, field, size_t(m.nc) * 8);
+        m.wls_gradient_device(t.data(), v.data(), condition_limit, df.as<double>(), dg.as<double>(),
+                              dfb.as<uint8_t>());
+        d2h(grad, dg.p, size_t(m.nc) * 24);
+        if (fallback_cells) {
+            std::vector<uint8_t> fb(m.nc);
+            d2h(fb.data(), dfb.p, m.nc);
+            size_t count = 0;
+            for (uint8_t b : fb) count += b;
+            *fallback_cells = count;
+        }
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_momentum_pattern(cfdxhip_fv* fv, size_t* n_rows, size_t* nnz, uint32_t* rowp,
+                                           uint32_t* cols) {
+    if (!fv || !n_rows || !nnz) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        const PressurePlan& pl = fv->mesh.pressure_plan(false, 0);
+        *n_rows = pl.n;
+        *nnz = pl.nnz;
+        if (rowp) std::memcpy(rowp, pl.h_rowp.data(), pl.h_rowp.size() * 4);
+        if (cols) std::memcpy(cols, pl.h_cols.data(), pl.h_cols.size() * 4);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_momentum_assemble_device(cfdxhip_fv* fv, const uint8_t* vbt, const double* vbv,
+                                                   const double* mu, const double* phi, const double* grad_p,
+                                                   const double* body, const double* U, int32_t bounded,
+                                                   double alpha_u, double* values, double* rhs, double* diag) {
+    if (!fv || !mu || !phi || !grad_p || !U || !values || !rhs || !diag || (fv->mesh.npatch && (!vbt || !vbv)))
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        fv->mesh.momentum_assemble_device(vbt, vbv, mu, phi, grad_p, body, U, bounded != 0, alpha_u, values, rhs,
+                                          diag);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_momentum_assemble(cfdxhip_fv* fv, const uint8_t* vbt, const double* vbv, const double* mu,
+                                            const double* phi, const double* grad_p, const double* body,
+                                            const double* U, int32_t bounded, double alpha_u, double* values,
+                                            double* rhs, double* diag) {
+    if (!fv || !mu || !phi || !grad_p || !U || !values || !rhs || !diag || (fv->mesh.npatch && (!vbt || !vbv)))
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const size_t nc = m.nc, nf = m.nf;
+        const PressurePlan& pl = m.pressure_plan(false, 0);
+        DevBuf dmu(nc * 8), dphi(nf * 8), dgp(nc * 24), dbody(body ? nc * 24 : 8), du(nc * 24), dv(pl.nnz * 8),
+            drhs(nc * 24), dd(nc * 8);
+        h2d(dmu.p, mu, nc * 8);
+        h2d(dphi.p, phi, nf * 8);
+        h2d(dgp.p, grad_p, nc * 24);
+        if (body) h2d(dbody.p, body, nc * 24);
+        h2d(du.p, U, nc * 24);
+        m.momentum_assemble_device(vbt, vbv, dmu.as<double>(), dphi.as<double>(), dgp.as<double>(),
+                                   body ? dbody.as<double>() : nullptr, du.as<double>(), bounded != 0, alpha_u,
+                                   dv.as<double>(), drhs.as<double>(), dd.as<double>());
+        d2h(values, dv.p, pl.nnz * 8);
+        d2h(rhs, drhs.p, nc * 24);
+        d2h(diag, dd.p, nc * 8);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_momentum_hbya_device(cfdxhip_fv* fv, const double* values, const double* rhs,
+                                               const double* diag, const double* grad_p, const double* U, double* rAU,
+                                               double* HbyA) {
+    if (!fv || !values || !rhs || !diag || !grad_p || !U || !rAU || !HbyA) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        fv->mesh.momentum_hbya_device(values, rhs, diag, grad_p, U, rAU, HbyA);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_momentum_hbya(cfdxhip_fv* fv, const double* values, const double* rhs, const double* diag,
+                                        const double* grad_p, const double* U, double* rAU, double* HbyA) {
+    if (!fv || !values || !rhs || !diag || !grad_p || !U || !rAU || !HbyA) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const size_t nc = m.nc;
+        const PressurePlan& pl = m.pressure_plan(false, 0);
+        DevBuf dv(pl.nnz * 8), drhs(nc * 24), dd(nc * 8), dgp(nc * 24), du(nc * 24), dr(nc * 24), dh(nc * 24);
+        h2d(dv.p, values, pl.nnz * 8);
+        h2d(drhs.p, rhs, nc * 24);
+        h2d(dd.p, diag, nc * 8);
+        h2d(dgp.p, grad_p, nc * 24);
+        h2d(du.p, U, nc * 24);
+        m.momentum_hbya_device(dv.as<double>(), drhs.as<double>(), dd.as<double>(), dgp.as<double>(),
+                               du.as<double>(), dr.as<double>(), dh.as<double>());
+        d2h(rAU, dr.p, nc * 24);
+        d2h(HbyA, dh.p, nc * 24);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_velocity_correct_device(cfdxhip_fv* fv, const double* HbyA, const double* rAU,
+                                                  const double* grad_p, double* U) {
+    if (!fv || !HbyA || !rAU || !grad_p || !U) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        fv->mesh.velocity_correct_device(HbyA, rAU, grad_p, U);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_velocity_correct(cfdxhip_fv* fv, const double* HbyA, const double* rAU,
+                                           const double* grad_p, double* U) {
+    if (!fv || !HbyA || !rAU || !grad_p || !U) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const size_t nc = m.nc;
+        DevBuf dh(nc * 24), dr(nc * 24), dg(nc * 24), du(nc * 24);
+        h2d(dh.p, HbyA, nc * 24);
+        h2d(dr.p, rAU, nc * 24);
+        h2d(dg.p, grad_p, nc * 24);
+        m.velocity_correct_device(dh.as<double>(), dr.as<double>(), dg.as<double>(), du.as<double>());
+        d2h(U, du.p, nc * 24);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_rhie_chow_flux_device(cfdxhip_fv* fv, const uint8_t* vbt, const double* vbv,
+                                                const uint8_t* pbt, const double* pbv, double rho, const double* HbyA,
+                                                const double* p, const double* rAU, double* phi) {
+    if (!fv || !HbyA || !p || !rAU || !phi || !std::isfinite(rho) ||
+        (fv->mesh.npatch && (!vbt || !vbv || !pbt || !pbv)))
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        fv->mesh.rhie_chow_flux_device(vbt, vbv, pbt, pbv, rho, HbyA, p, rAU, phi);
+        return CFDXHIP_OK;
+    });
+}
+
+cfdxhip_status cfdxhip_fv_rhie_chow_flux(cfdxhip_fv* fv, const uint8_t* vbt, const double* vbv, const uint8_t* pbt,
+                                         const double* pbv, double rho, const double* HbyA, const double* p,
+                                         const double* rAU, double* phi) {
+    if (!fv || !HbyA || !p || !rAU || !phi || !std::isfinite(rho) ||
+        (fv->mesh.npatch && (!vbt || !vbv || !pbt || !pbv)))
+        return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return fv_guarded(fv, [&]() {
+        FvMesh& m = fv->mesh;
+        const size_t nc = m.nc, nf = m.nf;
+        DevBuf dh(nc * 24), dp(nc * 8), dr(nc * 24), dphi(nf * 8);
+        h2d(dh.p, HbyA, nc * 24);
+        h2d(dp.p, p, nc * 8);
+        h2d(dr.p, rAU, nc * 24);
+        m.rhie_chow_flux_device(vbt, vbv, pbt, pbv, rho, dh.as<double>(), dp.as<double>(), dr.as<double>(),
+                                dphi.as<double>());
+        d2h(phi, dphi.p, nf * 8);
+        return CFDXHIP_OK;
+    });
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------ learned stage --

@@ -780,6 +780,14 @@ __global__ __launch_bounds__(kBlock) void scale_sqrt_copy_kernel(uint32_t n, con
     if (i < n) v[i] = w[i] / sqrt(*sq_ptr);
 }
 
+// x += y
+__global__ __launch_bounds__(kBlock) void add_into_kernel(uint32_t n, const double* __restrict__ y,
+                                                         double* __restrict__ x) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] += y[i];
+}
+
 // x += sum_j y_j Z_j (y on device)
 __global__ __launch_bounds__(kBlock) void basis_axpy_kernel(uint32_t n, uint32_t m, const double* __restrict__ Z,
                                                            const double* __restrict__ y, double* __restrict__ x) {
@@ -950,10 +958,6 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
         return true;
     };
     if (!std::isfinite(bn)) return finish(2, 0, bn);
-    if (bn == 0.0) {
-        err = "FGMRES with a zero right-hand side is not supported";
-        return finish(3, 0, 0.0);
-    }
     const double tolv = tol * bn;
     auto true_res = [&]() {
         launch_spmv<double>(L0.A, x, InitZero<double>{}, EpiResidual<double, true>{b, r, P0}, stream_);
@@ -961,6 +965,48 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
         fetch_scalars(1);
         return std::sqrt(h_scal_[0]);
     };
+    if (bn == 0.0) {
+        // Homogeneous system (core/linalg/gmres_solver.h:74-135, :180-196):
+        // with A x0 != 0, solve A y = -A x0 from y = 0 and certify x0 + y
+        // against ||A x0||; with A x0 == 0, x0 is the solution.
+        const double ax0 = true_res();  // r = -A x0 (solve ordering)
+        auto done = [&](uint32_t status, uint64_t it, double res, double rel) {
+            st.status = status;
+            st.iterations = it;
+            st.residual = res;
+            st.residual_relative = rel;
+            gather_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x, x_caller);
+            HIP_OK(hipStreamSynchronize(stream_));
+            st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            return true;
+        };
+        if (!std::isfinite(ax0)) return done(2, 0, ax0, ax0);
+        if (ax0 > 0.0) {
+            double* rc = need_vec(work_, 10, n);
+            double* yc = need_vec(work_, 11, n);
+            double* xc = need_vec(work_, 12, n);
+            gather_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, r, rc);
+            gather_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x, xc);
+            fill_kernel<double><<<g, kBlock, 0, stream_>>>(n, 0.0, yc);
+            SolveStats sh;
+            if (!fgmres(rc, yc, restart, max_iter, tol, mixed, sh, err, learned)) return false;
+            add_into_kernel<<<g, kBlock, 0, stream_>>>(n, yc, xc);
+            scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, xc, x);
+            fill_kernel<double><<<g, kBlock, 0, stream_>>>(n, 0.0, b);
+            const double res = true_res();
+            const double rel = res / ax0;
+            const bool certified = std::isfinite(res) && std::isfinite(rel) && rel <= tol;
+            done(certified ? 0u : (sh.status == 3 ? 3u : 2u), sh.iterations, res, rel);
+            st.reorthogonalisations = sh.reorthogonalisations;
+            st.pc_applications = sh.pc_applications;
+            return true;
+        }
+        dot_partial_kernel<double><<<gs, kBlock, 0, stream_>>>(n, x, x, P0);
+        reduce(P0, gs, 1, scal_);
+        fetch_scalars(1);
+        const double ref_scale = std::max({std::sqrt(h_scal_[0]), 1.0, 1e-300});
+        return done(ax0 <= tol * ref_scale ? 0u : 2u, 0, ax0, ax0 / ref_scale);
+    }
     if (learned && (!mixed || !learned->stage)) {
         err = "the learned stage runs inside mixed-precision FGMRES only";
         return false;

@@ -459,6 +459,686 @@ __global__ __launch_bounds__(kBlock) void soa_to_aos_kernel(uint32_t n, Soa3 s, 
     aos[3 * size_t(i) + 2] = s.z[i];
 }
 
+// ------------------------------------------------------ momentum side ---
+//
+// Reference anchors:
+//   compute_scalar_gradient_with_boundary   physics/finite_volume_transport.h:505-586
+//   weighted_least_squares_gradient_robust  core/fvm/least_squares_gradient.h:77-205
+//   compute_boundary_aware_wls_gradient     physics/finite_volume_transport.h:606-715
+//   compute_nonorthogonal_diffusion_face_fluxes                         :717-874
+//   assemble_scalar_equation (UPWIND, cell viscosity)                   :882-1310
+//   assemble_momentum_component             physics/steady_incompressible_solver.h:3262-3311
+//   relax_momentum_equation                 :3728-3770
+//   rAU / build_hbya                        :7816-7836 / :6430-6465
+//   velocity correction                     :9545-9557
+//   make_rhie_chow_mass_flux                :2063-2217
+//
+// The three velocity components share the flux, the viscosity and the
+// boundary-condition types, so their matrices are identical: one value array
+// (and one diagonal) serves all three, each component keeping its own RHS.
+// Face kernels compute per-face terms once; cell kernels gather them in
+// ascending face order, the order of the reference's global face loop, so the
+// per-cell sums are bit-identical (FMA contraction is off).
+
+constexpr int kMaxCellFaces = 10;  // WLS stencil: one sample per cell face
+
+// std::max(x, 0.0) and std::clamp(x, 0, 1) exactly (signed zeros included).
+__device__ __forceinline__ double max0(double x) { return x < 0.0 ? 0.0 : x; }
+__device__ __forceinline__ double clamp01(double v) { return v < 0.0 ? 0.0 : (1.0 < v ? 1.0 : v); }
+__device__ __forceinline__ double mag2_3(D3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+
+// boundary_normal_distance: finite_volume_transport.h:247-269.
+__device__ __forceinline__ bool boundary_normal_distance(D3 Sf, D3 Cf, D3 Cc, double V, double& out) {
+#pragma clang fp contract(off)
+    const double area = mag3(Sf);
+    if (!(area > 0.0) || !isfinite(area)) return false;
+    const double dn = fabs(dot3(sub3(Cf, Cc), Sf)) / area;
+    if (!(dn > 0.0) || !isfinite(dn)) {
+        if (V > 0.0 && isfinite(V)) {
+            const double dn2 = sqrt(V);
+            if (dn2 > 0.0 && isfinite(dn2)) {
+                out = dn2;
+                return true;
+            }
+        }
+        return false;
+    }
+    out = dn;
+    return true;
+}
+
+// K scalar fields at once (the velocity components share the stencil). Field
+// k's boundary value at patch p is bcv[p * bstride + k]; bct: 0 zero-gradient,
+// 1 fixed value, 2 fixed gradient. Green-Gauss first (kept where the WLS fit
+// is not certified), then the one-sided-Jacobi WLS fit.
+template <int K>
+__global__ __launch_bounds__(kBlock) void wls_gradient_kernel(
+    uint32_t nc, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cf,
+    const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh, const uint32_t* __restrict__ patch,
+    uint32_t npatch, const uint8_t* __restrict__ kind, const uint8_t* __restrict__ bct,
+    const double* __restrict__ bcv, uint32_t bstride, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
+    int dim, double limit, const double* __restrict__ field, double* __restrict__ grad,
+    uint8_t* __restrict__ fallback, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const size_t n1 = nc, n3 = 3 * size_t(nc);
+    const uint32_t k0 = coff[c], k1 = coff[c + 1];
+    const D3 centre = cc.get(c);
+    const double V = vol[c];
+    double value[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) value[k] = field[k * n1 + c];
+    // Green-Gauss with boundary values (compute_scalar_gradient_with_boundary).
+    {
+        D3 sum[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) sum[k] = d3(0.0, 0.0, 0.0);
+        for (uint32_t q = k0; q < k1; ++q) {
+            const uint32_t f = cf[q];
+            const bool own = owner[f] == c;
+            const D3 Sf = own ? fs.get(f) : mul3(fs.get(f), -1.0);
+            double fv[K];
+            if (neigh[f] >= 0) {
+                const uint32_t other = own ? uint32_t(neigh[f]) : owner[f];
+                const double dc = mag3(sub3(fc.get(f), centre));
+                const double dn = mag3(sub3(cc.get(other), fc.get(f)));
+                if (!(dc + dn > 0.0) || !isfinite(dc + dn)) {
+                    err[0] = 30;
+                    return;
+                }
+                const double w = dn / (dc + dn);
+#pragma unroll
+                for (int k = 0; k < K; ++k) fv[k] = w * value[k] + (1.0 - w) * field[k * n1 + other];
+            } else {
+                const uint32_t pp = patch[f];
+                if (pp < npatch && kind[pp] == 1) continue;
+                const uint8_t t = pp < npatch ? bct[pp] : 0;
+                double dn = 0.0;
+                if (t == 2 && !boundary_normal_distance(fs.get(f), fc.get(f), centre, V, dn)) {
+                    err[0] = 31;
+                    return;
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    fv[k] = value[k];
+                    if (t == 1) fv[k] = bcv[size_t(pp) * bstride + k];
+                    else if (t == 2) fv[k] = value[k] + bcv[size_t(pp) * bstride + k] * dn;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (!isfinite(fv[k])) {
+                    err[0] = 32;
+                    return;
+                }
+                sum[k] = add3(sum[k], mul3(Sf, fv[k]));
+            }
+        }
+        if (!(V > 0.0) || !isfinite(V)) {
+            err[0] = 33;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            grad[k * n3 + c] = sum[k].x / V;
+            grad[k * n3 + n1 + c] = sum[k].y / V;
+            grad[k * n3 + 2 * n1 + c] = sum[k].z / V;
+        }
+    }
+    // Weighted design matrix rows (1/|d| weights) and right-hand sides.
+    if (k1 - k0 > uint32_t(kMaxCellFaces)) {
+        err[0] = 34;
+        return;
+    }
+    double A[kMaxCellFaces][3];
+    double B[kMaxCellFaces][K];
+    bool ok[K];
+    bool geo_ok = isfinite(centre.x) && isfinite(centre.y) && isfinite(centre.z);
+#pragma unroll
+    for (int k = 0; k < K; ++k) ok[k] = isfinite(value[k]);
+    int ns = 0;
+    for (uint32_t q = k0; q < k1; ++q) {
+        const uint32_t f = cf[q];
+        const bool own = owner[f] == c;
+        D3 pt;
+        double sv[K];
+        if (neigh[f] >= 0) {
+            const uint32_t other = own ? uint32_t(neigh[f]) : owner[f];
+            pt = cc.get(other);
+#pragma unroll
+            for (int k = 0; k < K; ++k) sv[k] = field[k * n1 + other];
+        } else {
+            const uint32_t pp = patch[f];
+            if (pp < npatch && kind[pp] == 1) continue;
+            const uint8_t t = pp < npatch ? bct[pp] : 0;
+            const D3 sf = own ? fs.get(f) : mul3(fs.get(f), -1.0);
+            const double area = mag3(sf);
+            if (!(area > 0.0) || !isfinite(area)) {
+                err[0] = 35;
+                return;
+            }
+            const D3 normal = mul3(sf, 1.0 / area);
+            const double nd = dot3(sub3(fc.get(f), centre), normal);
+            if (!(nd > 0.0) || !isfinite(nd)) {
+                err[0] = 36;
+                return;
+            }
+            if (t == 1) {
+                pt = fc.get(f);
+#pragma unroll
+                for (int k = 0; k < K; ++k) sv[k] = bcv[size_t(pp) * bstride + k];
+            } else {
+                pt = add3(centre, mul3(normal, nd));
+#pragma unroll
+                for (int k = 0; k < K; ++k) sv[k] = value[k] + nd * (t == 2 ? bcv[size_t(pp) * bstride + k] : 0.0);
+            }
+        }
+        const D3 d = sub3(pt, centre);
+        const double r2 = mag2_3(d);
+        if (!(r2 > 0.0) || !isfinite(r2)) geo_ok = false;
+        const double factor = 1.0 / sqrt(r2);
+        if (!isfinite(factor)) geo_ok = false;
+#pragma unroll
+        for (int i = 0; i < kMaxCellFaces; ++i)
+            if (i == ns) {
+                A[i][0] = factor * d.x;
+                A[i][1] = factor * d.y;
+                A[i][2] = factor * d.z;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const double dv = sv[k] - value[k];
+                    if (!isfinite(dv)) ok[k] = false;
+                    B[i][k] = factor * dv;
+                }
+            }
+        ++ns;
+    }
+    bool certified = false;
+    double Vm[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    double sigma[3] = {0.0, 0.0, 0.0};
+    int order[3] = {0, 1, 2};
+    if (geo_ok) {
+        constexpr double eps = 2.220446049250313e-16;
+        for (int sweep = 0; sweep < 40; ++sweep) {
+            bool changed = false;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+#pragma unroll
+                for (int k = p + 1; k < 3; ++k) {
+                    double alpha = 0.0, beta = 0.0, gamma = 0.0;
+#pragma unroll
+                    for (int i = 0; i < kMaxCellFaces; ++i)
+                        if (i < ns) {
+                            alpha += A[i][p] * A[i][p];
+                            beta += A[i][k] * A[i][k];
+                            gamma += A[i][p] * A[i][k];
+                        }
+                    if (!(alpha > 0.0) || !(beta > 0.0)) continue;
+                    if (fabs(gamma) <= 8.0 * eps * sqrt(alpha * beta)) continue;
+                    const double tau = (beta - alpha) / (2.0 * gamma);
+                    const double t = copysign(1.0, tau) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double cs = 1.0 / sqrt(1.0 + t * t);
+                    const double sn = cs * t;
+#pragma unroll
+                    for (int i = 0; i < kMaxCellFaces; ++i)
+                        if (i < ns) {
+                            const double op = A[i][p], ok2 = A[i][k];
+                            A[i][p] = cs * op - sn * ok2;
+                            A[i][k] = sn * op + cs * ok2;
+                        }
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const double op = Vm[r][p], ok2 = Vm[r][k];
+                        Vm[r][p] = cs * op - sn * ok2;
+                        Vm[r][k] = sn * op + cs * ok2;
+                    }
+                    changed = true;
+                }
+            }
+            if (!changed) break;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double s2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < kMaxCellFaces; ++i)
+                if (i < ns) s2 += A[i][j] * A[i][j];
+            sigma[j] = sqrt(s2);
+        }
+        // Descending, ties in index order (the reference's three-element sort
+        // is an insertion sort).
+        if (sigma[order[1]] > sigma[order[0]]) {
+            const int tmp = order[0];
+            order[0] = order[1];
+            order[1] = tmp;
+        }
+        if (sigma[order[2]] > sigma[order[1]]) {
+            const int tmp = order[1];
+            order[1] = order[2];
+            order[2] = tmp;
+            if (sigma[order[1]] > sigma[order[0]]) {
+                const int tmp2 = order[0];
+                order[0] = order[1];
+                order[1] = tmp2;
+            }
+        }
+        const double scale = sigma[order[0]];
+        const double cutoff = 128.0 * eps * scale;
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (sigma[order[j]] > cutoff) ++rank;
+        if (dim == 2 && rank == 3 && sigma[order[2]] <= 1e-10 * sigma[order[0]]) rank = 2;
+        const bool full_rank = rank == dim;
+        const double min_pivot = full_rank ? sigma[order[dim - 1]] : 0.0;
+        const double cond = full_rank && min_pivot > 0.0 ? scale / min_pivot : INFINITY;
+        certified = rank == dim && full_rank && rank > 0 && isfinite(cond) && cond <= limit;
+    }
+    bool any_fallback = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        bool use = certified && ok[k];
+        double g[3] = {0.0, 0.0, 0.0};
+        if (use) {
+            for (int oi = 0; oi < dim; ++oi) {
+                const int j = order[oi];
+                double proj = 0.0;
+#pragma unroll
+                for (int i = 0; i < kMaxCellFaces; ++i)
+                    if (i < ns) proj += (j == 0 ? A[i][0] : (j == 1 ? A[i][1] : A[i][2])) * B[i][k];
+                const double sj = sigma[j];
+                const double coeff = proj / (sj * sj);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) g[r] += (j == 0 ? Vm[r][0] : (j == 1 ? Vm[r][1] : Vm[r][2])) * coeff;
+            }
+            use = isfinite(g[0]) && isfinite(g[1]) && isfinite(g[2]);
+        }
+        if (use) {
+            grad[k * n3 + c] = g[0];
+            grad[k * n3 + n1 + c] = g[1];
+            grad[k * n3 + 2 * n1 + c] = g[2];
+        } else {
+            any_fallback = true;
+        }
+    }
+    if (fallback) fallback[c] = any_fallback ? 1 : 0;
+}
+
+// Cell viscosity must be finite and non-negative; flags[0] = 1 when any is
+// positive (diffusion active).
+__global__ __launch_bounds__(kBlock) void viscosity_check_kernel(uint32_t nc, const double* __restrict__ mu,
+                                                                int* __restrict__ flags, int* __restrict__ err) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const double m = mu[c];
+    if (!(m >= 0.0) || !isfinite(m)) err[0] = 40;
+    if (m > 0.0) flags[0] = 1;
+}
+
+// needs_nonorthogonal_gradient scan (finite_volume_transport.h:987-1021):
+// flags[1] = 1 when an internal or fixed-value boundary face has a
+// tangential part above 1e-12 |S_f|.
+__global__ __launch_bounds__(kBlock) void nonorth_scan_kernel(uint32_t nf, const uint32_t* __restrict__ owner,
+                                                             const int32_t* __restrict__ neigh,
+                                                             const uint32_t* __restrict__ patch, uint32_t npatch,
+                                                             const uint8_t* __restrict__ kind,
+                                                             const uint8_t* __restrict__ vbt, Soa3 fc, Soa3 fs,
+                                                             Soa3 cc, int* __restrict__ flags, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const int32_t nr = neigh[f];
+    if (nr < 0) {
+        const uint32_t pp = patch[f];
+        if (pp < npatch && kind[pp] == 1) return;
+        if (!(pp < npatch && vbt[pp] == 1)) return;
+    }
+    const uint32_t o = owner[f];
+    const D3 sf = fs.get(f);
+    const double area = mag3(sf);
+    const D3 d = nr >= 0 ? sub3(cc.get(uint32_t(nr)), cc.get(o)) : sub3(fc.get(f), cc.get(o));
+    const double sd = dot3(sf, d);
+    const double d2 = dot3(d, d);
+    if (!(area > 0.0) || !(sd > 0.0) || !(d2 > 0.0) || !isfinite(area) || !isfinite(sd) || !isfinite(d2)) {
+        err[0] = 41;
+        return;
+    }
+    const double alpha = area * area / sd;
+    const D3 tangential = sub3(sf, mul3(d, alpha));
+    if (mag3(tangential) > 1e-12 * area) flags[1] = 1;
+}
+
+// Per face: the upwind/diffusion coefficients the face adds to the owner's
+// diagonal (aon) and to the neighbour's diagonal, or for a fixed-value
+// boundary face the coefficient of the boundary value (ano); with needs, the
+// tangential diffusion flux of each velocity component (nof, 3 x nf) from
+// the components' WLS gradients ugrad (3 components x 3 x nc).
+__global__ __launch_bounds__(kBlock) void momentum_face_kernel(
+    uint32_t nf, uint32_t nc, const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh,
+    const uint32_t* __restrict__ patch, uint32_t npatch, const uint8_t* __restrict__ kind,
+    const uint8_t* __restrict__ vbt, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
+    const double* __restrict__ mu, const double* __restrict__ phi, const double* __restrict__ ugrad, int bounded,
+    int needs, double* __restrict__ aon, double* __restrict__ ano, double* __restrict__ nof,
+    int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const size_t n1 = nc, n3 = 3 * size_t(nc);
+    const uint32_t o = owner[f];
+    const double F = phi[f];
+    if (!isfinite(F)) err[0] = 42;
+    double t_on = 0.0, t_no = 0.0;
+    double flux[3] = {0.0, 0.0, 0.0};
+    const D3 Sf = fs.get(f), Cf = fc.get(f), Co = cc.get(o);
+    if (neigh[f] >= 0) {
+        const uint32_t n = uint32_t(neigh[f]);
+        const D3 Cn = cc.get(n);
+        const double distance = mag3(sub3(Cn, Co));
+        const double area = mag3(Sf);
+        if (!(distance > 0.0) || !(area > 0.0)) err[0] = 43;
+        const double d_owner = fabs(dot3(sub3(Cf, Co), Sf)) / area;
+        const double d_neigh = fabs(dot3(sub3(Cn, Cf), Sf)) / area;
+        if (!(d_owner > 0.0) || !(d_neigh > 0.0) || !isfinite(d_owner) || !isfinite(d_neigh)) err[0] = 44;
+        const double go = mu[o], gn = mu[n];
+        double D = 0.0;
+        if (go > 0.0 && gn > 0.0) D = area / (d_owner / go + d_neigh / gn);
+        t_on = D + max0(F);
+        t_no = D + max0(-F);
+        if (needs) {
+            const D3 d = sub3(Cn, Co);
+            const double sd = dot3(Sf, d);
+            const double alpha = area * area / sd;
+            const D3 tangential = sub3(Sf, mul3(d, alpha));
+            if (mag3(tangential) > 1e-12 * area) {
+                const double area_projection = fabs(dot3(Sf, d)) / area;
+                if (!(area_projection > 0.0)) err[0] = 45;
+                double conductance = 0.0;
+                if (go > 0.0 && gn > 0.0) conductance = area / (d_owner / go + d_neigh / gn);
+                const double gamma_face = conductance / alpha;
+                const D3 line = sub3(Cn, Co);
+                const double d2_line = mag2_3(line);
+                if (!(d2_line > 0.0) || !isfinite(d2_line)) err[0] = 46;
+                const double w = 1.0 - clamp01(dot3(sub3(Cf, Co), line) / d2_line);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double* g = ugrad + k * n3;
+                    const D3 go3 = d3(g[o], g[n1 + o], g[2 * n1 + o]);
+                    const D3 gn3 = d3(g[n], g[n1 + n], g[2 * n1 + n]);
+                    const D3 gf = add3(mul3(go3, w), mul3(gn3, 1.0 - w));
+                    flux[k] = gamma_face * dot3(gf, tangential);
+                    if (!isfinite(flux[k])) err[0] = 47;
+                }
+            }
+        }
+    } else {
+        const uint32_t pp = patch[f];
+        if (!(pp < npatch && kind[pp] == 1)) {
+            const bool fixed = pp < npatch && vbt[pp] == 1;
+            const double area = mag3(Sf);
+            double distance = 0.0;
+            if (!boundary_normal_distance(Sf, Cf, Co, vol[o], distance) || !(distance > 0.0) || !(area > 0.0))
+                err[0] = 48;
+            if (fixed) {
+                const double D = mu[o] * area / distance;
+                t_on = D + max0(F);
+                t_no = D + max0(-F);
+                if (needs) {
+                    const D3 d = sub3(Cf, Co);
+                    const double sd = dot3(Sf, d);
+                    const double alpha = area * area / sd;
+                    const D3 tangential = sub3(Sf, mul3(d, alpha));
+                    if (mag3(tangential) > 1e-12 * area) {
+                        const double gamma_face = mu[o];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const double* g = ugrad + k * n3;
+                            const D3 gf = d3(g[o], g[n1 + o], g[2 * n1 + o]);
+                            flux[k] = gamma_face * dot3(gf, tangential);
+                            if (!isfinite(flux[k])) err[0] = 47;
+                        }
+                    }
+                }
+            } else {
+                t_on = bounded ? F : max0(F);
+            }
+        }
+    }
+    aon[f] = t_on;
+    ano[f] = t_no;
+    if (needs) {
+        nof[f] = flux[0];
+        nof[size_t(nf) + f] = flux[1];
+        nof[2 * size_t(nf) + f] = flux[2];
+    }
+}
+
+// Per cell, faces in ascending id: diagonal, flux divergence, deferred
+// non-orthogonal terms and boundary sources exactly as the reference's face
+// loop accumulates them; then the bounded-convection correction, sources and
+// the equation relaxation (alpha < 1). diag: the relaxed diagonal (shared by
+// the three components); rhs: 3 x nc.
+__global__ __launch_bounds__(kBlock) void momentum_cell_kernel(
+    uint32_t nc, uint32_t nf, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cfs,
+    const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh, const uint32_t* __restrict__ patch,
+    uint32_t npatch, const uint8_t* __restrict__ kind, const uint8_t* __restrict__ vbt,
+    const double* __restrict__ vbv, const double* __restrict__ vol, const double* __restrict__ phi,
+    const double* __restrict__ aon, const double* __restrict__ ano, const double* __restrict__ nof,
+    const double* __restrict__ gradp, const double* __restrict__ body, const double* __restrict__ U, int bounded,
+    int needs, int relax, double alpha, double* __restrict__ diag_out, double* __restrict__ rhs_out,
+    int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const size_t n1 = nc;
+    double diag = 0.0, div = 0.0;
+    double rhs[3] = {0.0, 0.0, 0.0}, def[3] = {0.0, 0.0, 0.0};
+    for (uint32_t q = coff[c]; q < coff[c + 1]; ++q) {
+        const uint32_t f = cfs[q];
+        const double F = phi[f];
+        if (neigh[f] >= 0) {
+            const bool own = owner[f] == c;
+            diag += own ? aon[f] : ano[f];
+            div = own ? div + F : div - F;
+            if (needs) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double t = nof[k * size_t(nf) + f];
+                    def[k] = own ? def[k] + t : def[k] - t;
+                }
+            }
+        } else {
+            const uint32_t pp = patch[f];
+            if (pp < npatch && kind[pp] == 1) continue;
+            if (needs) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) def[k] += nof[k * size_t(nf) + f];
+            }
+            diag += aon[f];
+            if (pp < npatch && vbt[pp] == 1) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) rhs[k] += ano[f] * vbv[3 * size_t(pp) + k];
+            } else if (!bounded && F < 0.0) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) def[k] -= F * U[k * n1 + c];
+            }
+            div += F;
+        }
+    }
+    if (bounded) diag -= div;
+    const double V = vol[c];
+    diag -= 0.0 * V;
+    if (!(diag > 0.0) || !isfinite(diag)) err[0] = 49;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double b = body ? body[k * n1 + c] : 0.0;
+        const double source = b - gradp[k * n1 + c];
+        rhs[k] += source * V;
+        rhs[k] += def[k];
+    }
+    if (relax) {
+        const double added = diag * (1.0 / alpha - 1.0);
+        diag += added;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rhs[k] += added * U[k * n1 + c];
+    }
+    diag_out[c] = diag;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rhs_out[k * n1 + c] = rhs[k];
+}
+
+// CSR values of the momentum matrix, one thread per row: the diagonal from
+// diag; off-diagonal (r, j) = 0 - the coefficient of each face joining r and j
+// in ascending face order (ano when r owns the face, else aon).
+__global__ __launch_bounds__(kBlock) void momentum_values_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                                const uint32_t* __restrict__ cols,
+                                                                const uint32_t* __restrict__ nzf_off,
+                                                                const uint32_t* __restrict__ nzf,
+                                                                const uint32_t* __restrict__ owner,
+                                                                const double* __restrict__ diag,
+                                                                const double* __restrict__ aon,
+                                                                const double* __restrict__ ano,
+                                                                double* __restrict__ vals) {
+#pragma clang fp contract(off)
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    for (uint32_t k = rowp[r]; k < rowp[r + 1]; ++k) {
+        if (cols[k] == r) {
+            vals[k] = diag[r];
+            continue;
+        }
+        double v = 0.0;
+        for (uint32_t q = nzf_off[k]; q < nzf_off[k + 1]; ++q) {
+            const uint32_t f = nzf[q];
+            v -= owner[f] == r ? ano[f] : aon[f];
+        }
+        vals[k] = v;
+    }
+}
+
+// rAU = 1 / diag and HbyA_k = rAU (rhs_k + grad_p_k V - sum_{j != r} a_rj U_k[j])
+// with the row's entries in storage order.
+__global__ __launch_bounds__(kBlock) void hbya_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                     const uint32_t* __restrict__ cols,
+                                                     const double* __restrict__ vals, const double* __restrict__ rhs,
+                                                     const double* __restrict__ diag,
+                                                     const double* __restrict__ gradp, const double* __restrict__ vol,
+                                                     const double* __restrict__ U, double* __restrict__ rau,
+                                                     double* __restrict__ hbya, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const size_t n1 = n;
+    const double a = diag[r];
+    if (!(a > 0.0) || !isfinite(a)) err[0] = 50;
+    const double ra = 1.0 / a;
+    double h[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h[k] = rhs[k * n1 + r] + gradp[k * n1 + r] * vol[r];
+    for (uint32_t j = rowp[r]; j < rowp[r + 1]; ++j) {
+        const uint32_t col = cols[j];
+        if (col == r) continue;
+        const double v = vals[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) h[k] -= v * U[k * n1 + col];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double hb = ra * h[k];
+        if (!isfinite(hb)) err[0] = 51;
+        rau[k * n1 + r] = ra;
+        hbya[k * n1 + r] = hb;
+    }
+}
+
+// U = HbyA - rAU grad_p V.
+__global__ __launch_bounds__(kBlock) void velocity_correct_kernel(uint32_t nc, const double* __restrict__ hbya,
+                                                                 const double* __restrict__ rau,
+                                                                 const double* __restrict__ gradp,
+                                                                 const double* __restrict__ vol,
+                                                                 double* __restrict__ U) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const size_t n1 = nc;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) U[k * n1 + c] = hbya[k * n1 + c] - rau[k * n1 + c] * gradp[k * n1 + c] * vol[c];
+}
+
+__global__ __launch_bounds__(kBlock) void positive_check_kernel(uint64_t n, const double* __restrict__ x,
+                                                               int code, int* __restrict__ err) {
+    const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    if (!isfinite(v) || v <= 0.0) err[0] = code;
+}
+
+// Rhie-Chow face mass flux from HbyA and p (make_rhie_chow_mass_flux,
+// HBYA_PREDICTOR contract, w = 1/2): rho H_f.S_f minus the pressure
+// correction (internal: rhie_chow_pressure_flux_internal; fixed-pressure
+// boundary: the boundary coefficient and deferred flux); EMPTY and SYMMETRY
+// faces carry no flux.
+__global__ __launch_bounds__(kBlock) void rhie_chow_flux_kernel(
+    uint32_t nf, uint32_t nc, const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh,
+    const uint32_t* __restrict__ patch, uint32_t npatch, const uint8_t* __restrict__ kind,
+    const uint8_t* __restrict__ vbt, const double* __restrict__ vbv, const uint8_t* __restrict__ pbt,
+    const double* __restrict__ pbv, double rho, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
+    const double* __restrict__ H, const double* __restrict__ p, const double* __restrict__ grad,
+    const double* __restrict__ rAU, double* __restrict__ phi, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const size_t n2 = 2 * size_t(nc);
+    const uint32_t o = owner[f];
+    D3 Hf = d3(H[o], H[nc + o], H[n2 + o]);
+    double correction = 0.0;
+    const D3 Sf = fs.get(f);
+    if (neigh[f] >= 0) {
+        const uint32_t n = uint32_t(neigh[f]);
+        const D3 Hn = d3(H[n], H[nc + n], H[n2 + n]);
+        const double wn = 0.5, wo = 1.0 - wn;
+        Hf = add3(mul3(Hf, 1.0 - wn), mul3(Hn, wn));
+        const D3 dAU = d3(wo * vol[o] * rAU[o] + wn * vol[n] * rAU[n],
+                          wo * vol[o] * rAU[nc + o] + wn * vol[n] * rAU[nc + n],
+                          wo * vol[o] * rAU[n2 + o] + wn * vol[n] * rAU[n2 + n]);
+        const D3 gp = d3(wo * grad[o] + wn * grad[n], wo * grad[nc + o] + wn * grad[nc + n],
+                         wo * grad[n2 + o] + wn * grad[n2 + n]);
+        const D3 dv = sub3(cc.get(n), cc.get(o));
+        double ic = 0.0, nofl = 0.0;
+        if (!ortho_coeff(Sf, dv, dAU, ic) || !nonortho_flux(Sf, dv, dAU, gp, nofl)) {
+            err[0] = 52;
+            return;
+        }
+        correction = rho * (ic * (p[n] - p[o]) + nofl);
+    } else {
+        const uint32_t pp = patch[f];
+        if (pp < npatch && (kind[pp] == 1 || kind[pp] == 2)) {
+            phi[f] = 0.0;
+            return;
+        }
+        if (pp < npatch) {
+            if (vbt[pp] == 1) Hf = d3(vbv[3 * pp], vbv[3 * pp + 1], vbv[3 * pp + 2]);
+            if (pbt[pp] == 1) {
+                const D3 dv = sub3(fc.get(f), cc.get(o));
+                const double d = mag3(dv);
+                if (!(d > 0.0) || !isfinite(d)) {
+                    err[0] = 52;
+                    return;
+                }
+                const D3 gp = d3(grad[o], grad[nc + o], grad[n2 + o]);
+                const D3 dAU = d3(vol[o] * rAU[o], vol[o] * rAU[nc + o], vol[o] * rAU[n2 + o]);
+                double ic = 0.0, def = 0.0;
+                if (!ortho_coeff(Sf, dv, dAU, ic) || !nonortho_flux(Sf, dv, dAU, gp, def)) {
+                    err[0] = 52;
+                    return;
+                }
+                correction = rho * (ic * (pbv[pp] - p[o]) + def);
+            }
+        }
+    }
+    phi[f] = rho * dot3(Hf, Sf) - correction;
+}
+
 }  // namespace
 
 // ==================================================================== FvMesh
@@ -467,13 +1147,15 @@ FvMesh::~FvMesh() { release(); }
 
 void FvMesh::release() {
     for (double** p : {&px, &py, &pz, &fcx, &fcy, &fcz, &fsx, &fsy, &fsz, &ccx, &ccy, &ccz, &vol, &work_coeff,
-                       &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi})
+                       &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi, &mom_aon,
+                       &mom_ano, &mom_nof, &mom_ugrad})
         dfree(*p);
     for (uint32_t** p : {&foff, &fvtx, &owner, &coff, &cfaces, &cfaces_sorted, &patch})
         dfree(*p);
     dfree(neigh);
     dfree(patch_kind);
     dfree(err);
+    dfree(mom_flags);
     for (auto* pl : {&pressure_full, &pressure_red})
         for (uint32_t** p : {&pl->rowp, &pl->cols, &pl->nz_row_diag, &pl->nzf_off, &pl->nzf, &pl->red_src,
                              &pl->red_rhs_src})
@@ -755,6 +1437,152 @@ const PressurePlan& FvMesh::diffusion_assemble_device(double gamma, const double
     HIP_OK(hipStreamSynchronize(stream));
     check_err("assemble_cell_diffusion_matrix: invalid geometry");
     return plan;
+}
+
+// ============================================================ momentum side
+
+void FvMesh::ensure_momentum_work(bool with_gradient) {
+    if (!mom_aon) {
+        mom_aon = dalloc<double>(nf);
+        mom_ano = dalloc<double>(nf);
+        mom_flags = dalloc<int>(4);
+    }
+    if (with_gradient && !mom_nof) {
+        mom_nof = dalloc<double>(3 * uint64_t(nf));
+        mom_ugrad = dalloc<double>(9 * uint64_t(nc));
+    }
+}
+
+int FvMesh::stencil_dimension() const {
+    for (uint8_t k : h_kind)
+        if (k == 1) return 2;
+    return 3;
+}
+
+void FvMesh::wls_gradient_device(const uint8_t* h_bt, const double* h_bv, double limit, const double* d_field,
+                                 double* d_grad, uint8_t* d_fallback) {
+    if (!(limit >= 1.0) || !std::isfinite(limit)) throw std::invalid_argument("boundary WLS: invalid condition limit");
+    uint8_t* d_bt = upload(h_bt, std::max<uint32_t>(npatch, 1));
+    double* d_bv = upload(h_bv, std::max<uint32_t>(npatch, 1));
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    wls_gradient_kernel<1><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
+                                                                patch_kind, d_bt, d_bv, 1, fc, fs, cc, vol,
+                                                                stencil_dimension(), limit, d_field, d_grad,
+                                                                d_fallback, err);
+    HIP_OK(hipStreamSynchronize(stream));
+    dfree(d_bt);
+    dfree(d_bv);
+    check_err("boundary WLS gradient: invalid geometry, boundary data or stencil size");
+}
+
+const PressurePlan& FvMesh::momentum_assemble_device(const uint8_t* h_vbt, const double* h_vbv, const double* d_mu,
+                                                     const double* d_phi, const double* d_gradp,
+                                                     const double* d_body, const double* d_U, bool bounded,
+                                                     double alpha, double* d_vals, double* d_rhs, double* d_diag) {
+    if (!(alpha > 0.0 && alpha <= 1.0)) throw std::invalid_argument("relax_momentum_equation: invalid relaxation");
+    const PressurePlan& plan = pressure_plan(false, 0);
+    ensure_momentum_work(false);
+    const uint32_t npp = std::max<uint32_t>(npatch, 1);
+    std::vector<uint8_t> vt(npp, 0);
+    std::vector<double> vv(3 * size_t(npp), 0.0);
+    for (uint32_t p = 0; p < npatch; ++p) {
+        vt[p] = h_vbt[p] == 1 ? 1 : 0;
+        for (int k = 0; k < 3; ++k) vv[3 * size_t(p) + k] = h_vbv[3 * size_t(p) + k];
+    }
+    uint8_t* d_vbt = upload(vt.data(), npp);
+    double* d_vbv = upload(vv.data(), vv.size());
+    const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    HIP_OK(hipMemsetAsync(mom_flags, 0, 4 * sizeof(int), stream));
+    viscosity_check_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, d_mu, mom_flags, err);
+    int flags[4] = {0, 0, 0, 0};
+    HIP_OK(hipMemcpyAsync(flags, mom_flags, sizeof flags, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    try {
+        check_err("assemble_momentum_component: cell viscosity must be finite and non-negative");
+        if (flags[0]) {
+            nonorth_scan_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, patch, npatch, patch_kind,
+                                                                     d_vbt, fc, fs, cc, mom_flags, err);
+            HIP_OK(hipMemcpyAsync(flags, mom_flags, sizeof flags, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            check_err("assemble_scalar_equation: invalid diffusion decomposition");
+        }
+        const bool needs = flags[0] && flags[1];
+        if (needs) {
+            ensure_momentum_work(true);
+            wls_gradient_kernel<3><<<grid_for(nc), kBlock, 0, stream>>>(
+                nc, coff, cfaces, owner, neigh, patch, npatch, patch_kind, d_vbt, d_vbv, 3, fc, fs, cc, vol,
+                stencil_dimension(), 1.e8, d_U, mom_ugrad, nullptr, err);
+        }
+        momentum_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind,
+                                                                  d_vbt, fc, fs, cc, vol, d_mu, d_phi, mom_ugrad,
+                                                                  bounded ? 1 : 0, needs ? 1 : 0, mom_aon, mom_ano,
+                                                                  mom_nof, err);
+        momentum_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(
+            nc, nf, coff, cfaces_sorted ? cfaces_sorted : cfaces, owner, neigh, patch, npatch, patch_kind, d_vbt,
+            d_vbv, vol, d_phi, mom_aon, mom_ano, mom_nof, d_gradp, d_body, d_U, bounded ? 1 : 0, needs ? 1 : 0,
+            alpha != 1.0 ? 1 : 0, alpha, d_diag, d_rhs, err);
+        momentum_values_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, plan.nzf_off, plan.nzf,
+                                                                    owner, d_diag, mom_aon, mom_ano, d_vals);
+        HIP_OK(hipStreamSynchronize(stream));
+        check_err("assemble_momentum_component: invalid flux, geometry, gradient or non-positive diagonal");
+    } catch (...) {
+        dfree(d_vbt);
+        dfree(d_vbv);
+        throw;
+    }
+    dfree(d_vbt);
+    dfree(d_vbv);
+    return plan;
+}
+
+void FvMesh::momentum_hbya_device(const double* d_vals, const double* d_rhs, const double* d_diag,
+                                  const double* d_gradp, const double* d_U, double* d_rau, double* d_hbya) {
+    const PressurePlan& plan = pressure_plan(false, 0);
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    hbya_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, d_vals, d_rhs, d_diag, d_gradp, vol,
+                                                     d_U, d_rau, d_hbya, err);
+    HIP_OK(hipStreamSynchronize(stream));
+    check_err("HbyA: invalid momentum diagonal or non-finite HbyA");
+}
+
+void FvMesh::velocity_correct_device(const double* d_hbya, const double* d_rau, const double* d_gradp,
+                                     double* d_U) {
+    velocity_correct_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, d_hbya, d_rau, d_gradp, vol, d_U);
+    HIP_OK(hipStreamSynchronize(stream));
+}
+
+void FvMesh::rhie_chow_flux_device(const uint8_t* h_vbt, const double* h_vbv, const uint8_t* h_pbt,
+                                   const double* h_pbv, double rho, const double* d_hbya, const double* d_p,
+                                   const double* d_rau, double* d_phi) {
+    ensure_work();
+    const uint32_t npp = std::max<uint32_t>(npatch, 1);
+    std::vector<uint8_t> vt(npp, 0), pt(npp, 0);
+    std::vector<double> vv(3 * size_t(npp), 0.0), pv(npp, 0.0);
+    for (uint32_t p = 0; p < npatch; ++p) {
+        vt[p] = h_vbt[p];
+        pt[p] = h_pbt[p];
+        pv[p] = h_pbv[p];
+        for (int k = 0; k < 3; ++k) vv[3 * size_t(p) + k] = h_vbv[3 * size_t(p) + k];
+    }
+    uint8_t* d_vbt = upload(vt.data(), npp);
+    uint8_t* d_pbt = upload(pt.data(), npp);
+    double* d_vbv = upload(vv.data(), vv.size());
+    double* d_pbv = upload(pv.data(), npp);
+    const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    positive_check_kernel<<<grid_for(3 * uint64_t(nc)), kBlock, 0, stream>>>(3 * uint64_t(nc), d_rau, 53, err);
+    gauss_gradient_device(d_pbt, d_pbv, d_p, work_grad);
+    rhie_chow_flux_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind,
+                                                               d_vbt, d_vbv, d_pbt, d_pbv, rho, fc, fs, cc, vol,
+                                                               d_hbya, d_p, work_grad, d_rau, d_phi, err);
+    HIP_OK(hipStreamSynchronize(stream));
+    dfree(d_vbt);
+    dfree(d_pbt);
+    dfree(d_vbv);
+    dfree(d_pbv);
+    check_err("make_rhie_chow_mass_flux: inverse momentum diagonal must be finite and positive, or invalid face");
 }
 
 }  // namespace cfdxhip

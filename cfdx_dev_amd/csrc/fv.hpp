@@ -1,4 +1,5 @@
-// Device-resident finite-volume mesh, geometry and pressure assembly.
+// Device-resident finite-volume mesh, geometry, pressure assembly and the
+// momentum side of the SIMPLE corrector.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,6 +47,10 @@ struct FvMesh {
     double *work_coeff = nullptr, *work_rc = nullptr, *work_bsrc = nullptr, *work_diag = nullptr,
            *work_rhs = nullptr, *work_vals = nullptr, *work_grad = nullptr, *work_phi = nullptr;
     uint8_t* fixed_face = nullptr;
+    // momentum side: per-face upwind/diffusion coefficients, tangential
+    // fluxes (3 x nf), velocity WLS gradients (3 x 3 x nc), flags
+    double *mom_aon = nullptr, *mom_ano = nullptr, *mom_nof = nullptr, *mom_ugrad = nullptr;
+    int* mom_flags = nullptr;
     PressurePlan pressure_full, pressure_red;
     // host copies of the addressing
     std::vector<uint32_t> h_owner, h_coff, h_cf, h_patch;
@@ -72,6 +77,22 @@ struct FvMesh {
                                                  uint64_t ref, double* d_vals, double* d_rhs);
     const PressurePlan& diffusion_assemble_device(double gamma, const double* d_source, const double* d_fval,
                                                   const uint8_t* d_fneu, double* d_vals, double* d_rhs);
+    // Momentum side (device pointers; velocity BCs per patch on the host:
+    // type 1 fixed value with 3 values, otherwise zero-gradient).
+    void wls_gradient_device(const uint8_t* h_bt, const double* h_bv, double limit, const double* d_field,
+                             double* d_grad, uint8_t* d_fallback);
+    const PressurePlan& momentum_assemble_device(const uint8_t* h_vbt, const double* h_vbv, const double* d_mu,
+                                                 const double* d_phi, const double* d_gradp, const double* d_body,
+                                                 const double* d_U, bool bounded, double alpha, double* d_vals,
+                                                 double* d_rhs, double* d_diag);
+    void momentum_hbya_device(const double* d_vals, const double* d_rhs, const double* d_diag,
+                              const double* d_gradp, const double* d_U, double* d_rau, double* d_hbya);
+    void velocity_correct_device(const double* d_hbya, const double* d_rau, const double* d_gradp, double* d_U);
+    void rhie_chow_flux_device(const uint8_t* h_vbt, const double* h_vbv, const uint8_t* h_pbt,
+                               const double* h_pbv, double rho, const double* d_hbya, const double* d_p,
+                               const double* d_rau, double* d_phi);
+    void ensure_momentum_work(bool with_gradient);
+    int stencil_dimension() const;
     void ensure_work();
     void check_err(const char* what);
     void release();

@@ -267,6 +267,72 @@ cfdxhip_status cfdxhip_fv_diffusion_assemble(cfdxhip_fv* fv, double gamma, const
                                              size_t* nnz, uint32_t* row_offsets, uint32_t* columns,
                                              double* values, double* rhs);
 
+/* ---------------------------------------- momentum side of SIMPLE ---- */
+
+/* compute_boundary_aware_wls_gradient (physics/finite_volume_transport.h:
+ * 606-715): per-cell weighted least-squares gradient of a scalar cell field
+ * (one-sided Jacobi fit, core/fvm/least_squares_gradient.h:77-205), kept at
+ * the boundary-aware Green-Gauss value (:505-586) where the fit is not
+ * rank- and condition-certified. bc_type per patch: 0 zero-gradient,
+ * 1 fixed value, 2 fixed gradient (bc_value holds the value or gradient).
+ * condition_limit: the reference's argument (1e8). grad: 3 x n_cells.
+ * fallback_cells (may be NULL): cells left at Green-Gauss. Cells may have at
+ * most 10 faces. Host pointers. */
+cfdxhip_status cfdxhip_fv_wls_gradient(cfdxhip_fv* fv, const uint8_t* bc_type, const double* bc_value,
+                                       double condition_limit, const double* field, double* grad,
+                                       size_t* fallback_cells);
+/* Sparsity of the momentum equations: every cell, its diagonal and its face
+ * neighbours, columns ascending (n_cells rows). NULL arrays query sizes. */
+cfdxhip_status cfdxhip_fv_momentum_pattern(cfdxhip_fv* fv, size_t* n_rows, size_t* nnz, uint32_t* row_offsets,
+                                           uint32_t* columns);
+/* The SIMPLE momentum predictor's three equations as the reference builds
+ * them: assemble_momentum_component (cell-viscosity overload,
+ * steady_incompressible_solver.h:3262-3311 over finite_volume_transport.h:
+ * 882-1310: UPWIND, bounded or not, orthogonal conductance plus the
+ * non-orthogonal correction from the WLS velocity gradients), then
+ * relax_momentum_equation (:3728-3770) with alpha_u in (0, 1]. vbc_type per
+ * patch: 1 fixed value (3 values per patch), otherwise zero-gradient. mu:
+ * cell viscosity; phi: face mass flux; grad_p: 3 x n_cells; body: 3 x n_cells
+ * force density or NULL; U: current velocity 3 x n_cells (convected field
+ * and relaxation state). The three component matrices are identical, so
+ * values (the momentum pattern) and diag (n_cells, relaxed) are written once;
+ * rhs is 3 x n_cells. Host pointers. */
+cfdxhip_status cfdxhip_fv_momentum_assemble(cfdxhip_fv* fv, const uint8_t* vbc_type, const double* vbc_value,
+                                            const double* mu, const double* phi, const double* grad_p,
+                                            const double* body, const double* U, int32_t bounded, double alpha_u,
+                                            double* values, double* rhs, double* diag);
+/* Same with device pointers (vbc arrays stay on the host). */
+cfdxhip_status cfdxhip_fv_momentum_assemble_device(cfdxhip_fv* fv, const uint8_t* vbc_type,
+                                                   const double* vbc_value, const double* mu, const double* phi,
+                                                   const double* grad_p, const double* body, const double* U,
+                                                   int32_t bounded, double alpha_u, double* values, double* rhs,
+                                                   double* diag);
+/* rAU = 1 / diag (steady_incompressible_solver.h:7816-7836, SIMPLE) and
+ * HbyA = rAU (rhs + grad_p V - sum of off-diagonal a U) (build_hbya
+ * :6430-6465), from the assembled equations and the predicted velocity U.
+ * rAU, HbyA: 3 x n_cells. Host pointers. */
+cfdxhip_status cfdxhip_fv_momentum_hbya(cfdxhip_fv* fv, const double* values, const double* rhs, const double* diag,
+                                        const double* grad_p, const double* U, double* rAU, double* HbyA);
+cfdxhip_status cfdxhip_fv_momentum_hbya_device(cfdxhip_fv* fv, const double* values, const double* rhs,
+                                               const double* diag, const double* grad_p, const double* U, double* rAU,
+                                               double* HbyA);
+/* Post-solve velocity U = HbyA - rAU grad_p V (steady_incompressible_solver.h:
+ * 9545-9557), grad_p of the corrected pressure. Host / device pointers. */
+cfdxhip_status cfdxhip_fv_velocity_correct(cfdxhip_fv* fv, const double* HbyA, const double* rAU,
+                                           const double* grad_p, double* U);
+cfdxhip_status cfdxhip_fv_velocity_correct_device(cfdxhip_fv* fv, const double* HbyA, const double* rAU,
+                                                  const double* grad_p, double* U);
+/* Post-solve face flux: make_rhie_chow_mass_flux (steady_incompressible_solver.h:
+ * 2063-2217) with the corrector's arguments (HBYA_PREDICTOR, linear weights,
+ * Gauss pressure gradient). Host / device pointers (BC arrays on the host). */
+cfdxhip_status cfdxhip_fv_rhie_chow_flux(cfdxhip_fv* fv, const uint8_t* vbc_type, const double* vbc_value,
+                                         const uint8_t* pbc_type, const double* pbc_value, double rho,
+                                         const double* HbyA, const double* p, const double* rAU, double* phi);
+cfdxhip_status cfdxhip_fv_rhie_chow_flux_device(cfdxhip_fv* fv, const uint8_t* vbc_type, const double* vbc_value,
+                                                const uint8_t* pbc_type, const double* pbc_value, double rho,
+                                                const double* HbyA, const double* p, const double* rAU,
+                                                double* phi);
+
 /* ------------------------------------------------------------------------
  * Learned preconditioner stage. No reference counterpart (the reference has
  * no learned component). A 4-layer message-passing network, hidden width 32,

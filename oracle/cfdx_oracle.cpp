@@ -1597,8 +1597,48 @@ int oracle_fgmres(void* hp, const double* b, double* x, int restart, size_t max_
     int cur = std::clamp(restart, rmin, rmax);
     if (n < size_t(cur)) cur = int(n);
     const double bn = knorm2(b, n);
-    if (!std::isfinite(bn) || bn == 0.0) {
-        res->status = 3;
+    if (!std::isfinite(bn)) {
+        res->status = 2;
+        return 0;
+    }
+    if (bn == 0.0) {
+        // Homogeneous system (gmres_solver.h:74-135): with A x0 != 0 solve the
+        // shifted system A y = -A x0 from y = 0 and certify x0 + y against
+        // ||A x0||; with A x0 == 0 the iterate is accepted below.
+        std::vector<double> ax0(n);
+        matvec(A, x, ax0.data());
+        const double ax0n = knorm2(ax0.data(), n);
+        if (!std::isfinite(ax0n)) {
+            *res = oracle_result{2, 0, 0, ax0n, ax0n, 0};
+            return 0;
+        }
+        if (ax0n > 0.0) {
+            std::vector<double> rhs(n), y(n, 0.0), ax(n), tr(n);
+            for (size_t i = 0; i < n; ++i) rhs[i] = -ax0[i];
+            oracle_result sh{};
+            oracle_fgmres(hp, rhs.data(), y.data(), restart, max_iter, tol, &sh);
+            for (size_t i = 0; i < n; ++i) x[i] += y[i];
+            matvec(A, x, ax.data());
+            for (size_t i = 0; i < n; ++i) tr[i] = b[i] - ax[i];
+            const double r = knorm2(tr.data(), n);
+            const double rel = r / ax0n;
+            const bool certified = std::isfinite(r) && std::isfinite(rel) && rel <= tol;
+            res->status = certified ? 0 : (sh.status == 3 ? 3 : 2);
+            res->iterations = sh.iterations;
+            res->residual = r;
+            res->residual_relative = rel;
+            res->residual_replacements = sh.residual_replacements;
+            return 0;
+        }
+        std::vector<double> ax(n), tr(n);
+        matvec(A, x, ax.data());
+        for (size_t i = 0; i < n; ++i) tr[i] = b[i] - ax[i];
+        const double beta0 = knorm2(tr.data(), n);
+        const double ref_scale = std::max({knorm2(x, n), 1.0, 1e-300});
+        res->status = beta0 <= tol * ref_scale ? 0 : 2;
+        res->iterations = 0;
+        res->residual = beta0;
+        res->residual_relative = beta0 / ref_scale;
         return 0;
     }
     const double rhs_scale = std::max(bn, 1e-300);

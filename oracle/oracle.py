@@ -34,6 +34,13 @@ class OracleResult(C.Structure):
                 ("residual_replacements", C.c_uint64)]
 
 
+def _cp(a):
+    """ctypes argument for an int (size) or a numpy array (pointer)."""
+    if isinstance(a, (int, np.integer)):
+        return C.c_size_t(int(a))
+    return a.ctypes.data_as(vp)
+
+
 def reference_available() -> bool:
     return os.path.exists(REF_SO)
 
@@ -209,6 +216,96 @@ class Oracle:
             self.lib.oracle_amg_smoother.argtypes = [vp, C.c_int, C.c_int, C.c_double]
             self.lib.oracle_amg_smoother(a.h, {"chebyshev": 1}[smoother], chebyshev_degree, chebyshev_ratio)
         return a
+
+    # ------------------------------------------------------- momentum side --
+    def _fv_args(self, m, geom):
+        """Mesh arguments of the momentum-side calls: the oracle takes the
+        mesh addressing plus geometry, the reference harness the raw mesh
+        (it builds the geometry itself)."""
+        if self.reference:
+            return self._mesh_args(m) + [m.patch_of_face, m.n_patches, m.patch_kind]
+        fc, fs, cc, vol = geom
+        return [m.n_faces, m.owner, m.neighbour, m.n_cells, m.cell_offsets, m.cell_faces, fc, fs, cc, vol,
+                m.patch_of_face, m.n_patches, m.patch_kind]
+
+    def wls_gradient(self, m, geom, bc_type, bc_value, field, condition_limit=1e8):
+        """compute_boundary_aware_wls_gradient; returns (grad 3 x n, fallback cells)."""
+        g = np.empty(3 * m.n_cells)
+        fb = sz()
+        fn = getattr(self.lib, self.p + "wls_gradient")
+        fn.restype = C.c_int
+        fn.argtypes = None
+        args = self._fv_args(m, geom) + [_f(bc_type, np.uint8), _f(bc_value, np.float64), _f(field, np.float64)]
+        st = fn(*[_cp(a) for a in args], C.c_double(condition_limit), _cp(g), C.byref(fb))
+        if st != 0:
+            raise RuntimeError("wls gradient failed")
+        return g, fb.value
+
+    def momentum_assemble(self, m, geom, vbc_type, vbc_value, mu, phi, grad_p, body, U, bounded=True,
+                          alpha=1.0):
+        """The three relaxed momentum equations: dict(rowp, cols, vals (3, nnz), rhs (3, n), diag (3, n))."""
+        fn = getattr(self.lib, self.p + "momentum_assemble")
+        fn.restype = C.c_int
+        nc = m.n_cells
+        args = self._fv_args(m, geom) + [_f(vbc_type, np.uint8), _f(vbc_value, np.float64), _f(mu, np.float64),
+                                         _f(phi, np.float64), _f(grad_p, np.float64)]
+        tail = [_f(U, np.float64)]
+        bp = None if body is None else _f(body, np.float64)
+        nnz = sz()
+        call = [_cp(a) for a in args] + [None if bp is None else _cp(bp)] + [_cp(a) for a in tail]
+        if fn(*call, C.c_int(int(bounded)), C.c_double(alpha), C.byref(nnz), None, None, None, None, None) != 0:
+            raise RuntimeError("momentum assembly failed")
+        rp = np.empty(nc + 1, np.uint32)
+        cols = np.empty(nnz.value, np.uint32)
+        vals = np.empty(3 * nnz.value)
+        rhs = np.empty(3 * nc)
+        diag = np.empty(3 * nc)
+        if fn(*call, C.c_int(int(bounded)), C.c_double(alpha), C.byref(nnz), _cp(rp), _cp(cols), _cp(vals),
+              _cp(rhs), _cp(diag)) != 0:
+            raise RuntimeError("momentum assembly failed")
+        return dict(rowp=rp, cols=cols, vals=vals.reshape(3, -1), rhs=rhs.reshape(3, -1), diag=diag.reshape(3, -1))
+
+    def momentum_hbya(self, rowp, cols, vals, rhs, diag, grad_p, vol, U):
+        """rAU and HbyA (3 x n each) from the relaxed equations and the predicted U."""
+        nc = len(rowp) - 1
+        rau = np.empty(3 * nc)
+        hbya = np.empty(3 * nc)
+        fn = self.lib.oracle_momentum_hbya
+        fn.restype = C.c_int
+        args = [_f(rowp, np.uint32), _f(cols, np.uint32), _f(np.ravel(vals), np.float64),
+                _f(np.ravel(rhs), np.float64), _f(np.ravel(diag), np.float64), _f(grad_p, np.float64),
+                _f(vol, np.float64), _f(U, np.float64)]
+        if fn(C.c_size_t(nc), *[_cp(a) for a in args], _cp(rau), _cp(hbya)) != 0:
+            raise RuntimeError("HbyA failed")
+        return rau, hbya
+
+    def velocity_correct(self, hbya, rau, grad_p, vol):
+        nc = len(vol)
+        U = np.empty(3 * nc)
+        args = [_f(hbya, np.float64), _f(rau, np.float64), _f(grad_p, np.float64), _f(vol, np.float64)]
+        self.lib.oracle_velocity_correct(C.c_size_t(nc), *[_cp(a) for a in args], _cp(U))
+        return U
+
+    def rhie_chow_flux(self, m, geom, vbc_type, vbc_value, pbc_type, pbc_value, rho, hbya, p, rau, grad_p=None):
+        """make_rhie_chow_mass_flux. The oracle takes the pressure's Gauss
+        gradient (oracle_gauss_gradient); the reference computes it."""
+        phi = np.empty(m.n_faces)
+        if self.reference:
+            fn = self.lib.ref_rhie_chow_flux
+            args = self._fv_args(m, geom) + [_f(vbc_type, np.uint8), _f(vbc_value, np.float64),
+                                             _f(pbc_type, np.uint8), _f(pbc_value, np.float64)]
+            tail = [_f(hbya, np.float64), _f(p, np.float64), _f(rau, np.float64)]
+        else:
+            fc, fs, cc, vol = geom
+            fn = self.lib.oracle_rhie_chow_flux
+            args = [m.n_faces, m.owner, m.neighbour, m.n_cells, fc, fs, cc, vol, m.patch_of_face, m.n_patches,
+                    m.patch_kind, _f(vbc_type, np.uint8), _f(vbc_value, np.float64), _f(pbc_type, np.uint8),
+                    _f(pbc_value, np.float64)]
+            tail = [_f(hbya, np.float64), _f(p, np.float64), _f(grad_p, np.float64), _f(rau, np.float64)]
+        fn.restype = C.c_int
+        if fn(*[_cp(a) for a in args], C.c_double(rho), *[_cp(a) for a in tail], _cp(phi)) != 0:
+            raise RuntimeError("Rhie-Chow flux failed")
+        return phi
 
     def chebyshev_coefficients(self, lam, ratio, degree):
         keep, scale = np.empty(degree), np.empty(degree)

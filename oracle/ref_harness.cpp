@@ -19,6 +19,10 @@
 //   NativeAMGPreconditioner           core/linalg/hypre_amg.cpp:163-330
 //   solve_cg_prepared                 core/linalg/cg_solver.h:653-667
 //   solve_fgmres                      core/linalg/gmres_solver.h:678-692
+//   compute_boundary_aware_wls_gradient physics/finite_volume_transport.h:606-715
+//   assemble_momentum_component       physics/steady_incompressible_solver.h:3262-3311
+//   relax_momentum_equation           physics/steady_incompressible_solver.h:3728-3770
+//   make_rhie_chow_mass_flux          physics/steady_incompressible_solver.h:2063-2217
 //
 // The SIMPLE corrector's matrix/RHS loop (steady_incompressible_solver.h
 // :8080-8181, :8599-8652) is inline in a 16k-line driver function and cannot
@@ -390,6 +394,158 @@ int ref_pressure_assemble(size_t np, const double* px, const double* py, const d
         return 0;
     } catch (const std::exception& e) {
         std::cerr << "ref_pressure_assemble: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+// Scalar boundary conditions per patch (name "patch<k>"): type 1 fixed
+// value, 2 fixed gradient, otherwise zero-gradient.
+static cfdx::physics::ScalarBoundaryConditions scalar_bcs(size_t n_patches, const uint8_t* t, const double* v,
+                                                         size_t stride, size_t comp) {
+    using namespace cfdx::physics;
+    ScalarBoundaryConditions bcs;
+    for (size_t p = 0; p < n_patches; ++p) {
+        ScalarBoundaryCondition bc;
+        if (t[p] == 1) {
+            bc.type = ScalarBoundaryType::FIXED_VALUE;
+            bc.value = v[stride * p + comp];
+        } else if (t[p] == 2) {
+            bc.type = ScalarBoundaryType::FIXED_GRADIENT;
+            bc.gradient = v[stride * p + comp];
+        }
+        bcs["patch" + std::to_string(p)] = bc;
+    }
+    return bcs;
+}
+
+// compute_boundary_aware_wls_gradient (physics/finite_volume_transport.h:606-715).
+int ref_wls_gradient(size_t np, const double* px, const double* py, const double* pz, size_t nf,
+                     const uint32_t* foff, const uint32_t* fvtx, const uint32_t* owner, const int32_t* neigh,
+                     size_t nc, const uint32_t* coff, const uint32_t* cfaces, const uint32_t* patch_of_face,
+                     size_t n_patches, const uint8_t* patch_kind, const uint8_t* bc_type, const double* bc_value,
+                     const double* field, double condition_limit, double* grad, size_t* fallback_cells) {
+    using namespace cfdx::physics;
+    try {
+        Mesh m = build_mesh(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces, patch_of_face,
+                            n_patches, patch_kind);
+        const FvGeometry g = build_fv_geometry(m);
+        const auto bcs = scalar_bcs(n_patches, bc_type, bc_value, 1, 0);
+        Field<double, Location::CELL> fld(nc, "phi", "", 1);
+        for (size_t c = 0; c < nc; ++c) fld(c) = field[c];
+        BoundaryWlsDiagnostics diag;
+        const auto gr = compute_boundary_aware_wls_gradient(fld, m, g, bcs, nullptr, nullptr, &diag,
+                                                            condition_limit);
+        for (int d = 0; d < 3; ++d)
+            for (size_t c = 0; c < nc; ++c) grad[d * nc + c] = gr.component_data(d)[c];
+        if (fallback_cells) *fallback_cells = diag.fallback_cells;
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_wls_gradient: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+// The SIMPLE predictor's three momentum equations: assemble_momentum_component
+// (cell-viscosity overload, steady_incompressible_solver.h:3262-3311) called as
+// the driver calls it (:7173-7193, laminar: no wall viscosity override, UPWIND,
+// the current velocity component as convected field), then
+// relax_momentum_equation (:3728-3770) as at :7767-7769.
+int ref_momentum_assemble(size_t np, const double* px, const double* py, const double* pz, size_t nf,
+                          const uint32_t* foff, const uint32_t* fvtx, const uint32_t* owner, const int32_t* neigh,
+                          size_t nc, const uint32_t* coff, const uint32_t* cfaces, const uint32_t* patch_of_face,
+                          size_t n_patches, const uint8_t* patch_kind, const uint8_t* vbc_type,
+                          const double* vbc_value, const double* mu, const double* phi, const double* grad_p,
+                          const double* body, const double* U, int bounded, double alpha, size_t* nnz_out,
+                          uint32_t* rowp, uint32_t* cols, double* vals, double* rhs, double* diag) {
+    using namespace cfdx::physics;
+    try {
+        Mesh m = build_mesh(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces, patch_of_face,
+                            n_patches, patch_kind);
+        const FvGeometry g = build_fv_geometry(m);
+        Field<double, Location::FACE> flux(nf, "phi", "kg/s", 1);
+        for (size_t f = 0; f < nf; ++f) flux(f) = phi[f];
+        Field<double, Location::CELL> gp(nc, "grad_p", "", 3);
+        for (int d = 0; d < 3; ++d)
+            for (size_t c = 0; c < nc; ++c) gp.component_data(d)[c] = grad_p[d * nc + c];
+        const std::vector<double> mu_eff(mu, mu + nc);
+        std::vector<uint8_t> vt(n_patches);
+        for (size_t p = 0; p < n_patches; ++p) vt[p] = vbc_type[p] == 1 ? 1 : 0;
+        for (int d = 0; d < 3; ++d) {
+            Field<double, Location::CELL> bodyd(nc, "body", "N/m3", 1), ud(nc, "U", "m/s", 1);
+            Vector uold(nc, 0.0);
+            for (size_t c = 0; c < nc; ++c) {
+                bodyd(c) = body ? body[d * nc + c] : 0.0;
+                ud(c) = U[d * nc + c];
+                uold(c) = U[d * nc + c];
+            }
+            const auto ubc = scalar_bcs(n_patches, vt.data(), vbc_value, 3, size_t(d));
+            auto eq = assemble_momentum_component(m, g, flux, gp, bodyd, mu_eff, ubc, size_t(d), bounded != 0,
+                                                  ConvectionScheme::UPWIND, &ud, nullptr, -1.0, 1.0, nullptr,
+                                                  nullptr);
+            relax_momentum_equation(eq, uold, alpha);
+            const size_t nnz = eq.matrix.nnz();
+            *nnz_out = nnz;
+            if (!rowp) continue;
+            std::copy(eq.matrix.row_offsets_data(), eq.matrix.row_offsets_data() + nc + 1, rowp);
+            std::copy(eq.matrix.columns_data(), eq.matrix.columns_data() + nnz, cols);
+            std::copy(eq.matrix.values_data(), eq.matrix.values_data() + nnz, vals + d * nnz);
+            for (size_t c = 0; c < nc; ++c) {
+                rhs[d * nc + c] = eq.rhs(c);
+                diag[d * nc + c] = eq.diagonal[c];
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_momentum_assemble: " << e.what() << "\n";
+        return 1;
+    }
+}
+
+// make_rhie_chow_mass_flux (steady_incompressible_solver.h:2063-2217) with the
+// SIMPLE corrector's arguments: HBYA_PREDICTOR, no skew-corrected gradient,
+// linear (non-projected) weights, no face velocity conditions.
+int ref_rhie_chow_flux(size_t np, const double* px, const double* py, const double* pz, size_t nf,
+                       const uint32_t* foff, const uint32_t* fvtx, const uint32_t* owner, const int32_t* neigh,
+                       size_t nc, const uint32_t* coff, const uint32_t* cfaces, const uint32_t* patch_of_face,
+                       size_t n_patches, const uint8_t* patch_kind, const uint8_t* vbc_type,
+                       const double* vbc_value, const uint8_t* pbc_type, const double* pbc_value, double rho,
+                       const double* HbyA, const double* p_in, const double* rAU, double* phi) {
+    using namespace cfdx::physics;
+    try {
+        Mesh m = build_mesh(np, px, py, pz, nf, foff, fvtx, owner, neigh, nc, coff, cfaces, patch_of_face,
+                            n_patches, patch_kind);
+        const FvGeometry g = build_fv_geometry(m);
+        ScalarBoundaryConditions pbcs;
+        VelocityBoundaryConditions vbcs;
+        for (size_t p = 0; p < n_patches; ++p) {
+            const std::string name = "patch" + std::to_string(p);
+            if (pbc_type[p] == 1) {
+                ScalarBoundaryCondition bc;
+                bc.type = ScalarBoundaryType::FIXED_VALUE;
+                bc.value = pbc_value[p];
+                pbcs[name] = bc;
+            }
+            if (vbc_type[p] == 1) {
+                VelocityBoundaryCondition bc;
+                bc.type = VelocityBoundaryCondition::Type::FIXED_VALUE;
+                bc.value = Vec3{vbc_value[3 * p], vbc_value[3 * p + 1], vbc_value[3 * p + 2]};
+                vbcs[name] = bc;
+            }
+        }
+        Field<double, Location::CELL> H(nc, "HbyA", "m/s", 3), p(nc, "p", "Pa", 1);
+        for (size_t c = 0; c < nc; ++c) {
+            p(c) = p_in[c];
+            for (int d = 0; d < 3; ++d) H.component_data(d)[c] = HbyA[d * nc + c];
+        }
+        std::array<std::vector<double>, 3> rau;
+        for (int d = 0; d < 3; ++d) rau[d].assign(rAU + d * nc, rAU + (d + 1) * nc);
+        const auto flux = make_rhie_chow_mass_flux(m, g, H, p, rau, rho, vbcs, pbcs,
+                                                   RhieChowVelocityContract::HBYA_PREDICTOR, false, false,
+                                                   nullptr);
+        for (size_t f = 0; f < nf; ++f) phi[f] = flux(f);
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ref_rhie_chow_flux: " << e.what() << "\n";
         return 1;
     }
 }

@@ -94,3 +94,47 @@ def test_gpu_vcycle_and_fgmres_nonsymmetric(oracle):
     y = oracle.spmv(s["rowp"], s["cols"], s["vals"], xg)
     assert np.linalg.norm(s["b"] - y) <= 1.01 * tol * np.linalg.norm(s["b"])
     assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
+def test_fgmres_zero_rhs_matches_reference(oracle, reference):
+    """Homogeneous systems (core/linalg/gmres_solver.h:74-135): x0 = 0 is
+    accepted at once; a nonzero x0 is driven to the null solution through the
+    shifted solve and certified against ||A x0||."""
+    s = schur_like(oracle, (8, 7, 6), 11)
+    ao = oracle.amg("boomer", require_certificate=False)
+    ar = reference.amg("boomer", require_certificate=False)
+    assert ao.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    assert ar.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    b = np.zeros(s["n"])
+    xo, xr = np.zeros(s["n"]), np.zeros(s["n"])
+    ro, rr = ao.fgmres(b, xo, 30, 300, 1e-10), ar.fgmres(b, xr, 30, 300, 1e-10)
+    assert (ro.status, ro.iterations, rr.status, rr.iterations) == (0, 0, 0, 0)
+    assert not xo.any() and not xr.any()
+    x0 = np.random.default_rng(5).normal(size=s["n"])
+    xo, xr = x0.copy(), x0.copy()
+    ro, rr = ao.fgmres(b, xo, 30, 300, 1e-10), ar.fgmres(b, xr, 30, 300, 1e-10)
+    assert (ro.status, ro.iterations) == (rr.status, rr.iterations) and ro.status == 0
+    assert ro.residual_relative == rr.residual_relative
+    assert np.array_equal(xo, xr)
+    assert np.abs(xo).max() < 1e-8 * np.abs(x0).max()
+
+
+@pytest.mark.gpu
+def test_gpu_fgmres_zero_rhs(oracle):
+    s = schur_like(oracle, (12, 10, 9), 12)
+    ao = oracle.amg("boomer", require_certificate=False)
+    assert ao.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    h = cx.NativeAMGPreconditioner(method="boomer", require_smoother_certificate=False)
+    assert h.setup(s["n"], s["rowp"], s["cols"], s["vals"]), h.last_error()
+    b = np.zeros(s["n"])
+    x = np.zeros(s["n"])
+    r = h.solve_fgmres(b, x, 30, 300, 1e-10)
+    assert (r.status, r.iterations) == (0, 0) and not x.any()
+    x0 = np.random.default_rng(6).normal(size=s["n"])
+    xg, xo = x0.copy(), x0.copy()
+    r = h.solve_fgmres(b, xg, 30, 300, 1e-10)
+    ro = ao.fgmres(b, xo, 30, 300, 1e-10)
+    assert r.status == 0 and ro.status == 0
+    assert r.iterations == ro.iterations
+    assert r.residual_relative <= 1e-10
+    assert np.abs(xg - xo).max() <= 1e-10 * np.abs(x0).max()
