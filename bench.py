@@ -109,6 +109,99 @@ def cpu_baseline(sample_n: int, tol: float, filt: float):
                 setup_s=t_setup, iterations=int(r.iterations), cpu=cpu_model(), nproc=os.cpu_count())
 
 
+def momentum_bytes(m, nnz):
+    """Algorithmic bytes of the momentum side on mesh m (each input and output
+    array touched once; the per-face and per-cell intermediates are excluded):
+    assembly of the three relaxed momentum equations, rAU/HbyA, the
+    Rhie-Chow flux (with its pressure gradient) and the velocity correction."""
+    n, F = m.n_cells, m.n_faces
+    Fi = m.n_internal_faces
+    Fb = F - Fi
+    L = len(m.cell_faces)
+    assembly = (8 * F + 4 * Fb + 48 * F + 32 * n + 4 * (n + 1) + 4 * L + 8 * n + 8 * F + 24 * n + 24 * n
+                + 4 * (n + 1) + 4 * nnz + 4 * (nnz + 1) + 8 * Fi + 8 * nnz + 24 * n + 8 * n)
+    hbya = 4 * (n + 1) + 12 * nnz + 24 * n + 8 * n + 24 * n + 8 * n + 24 * n + 48 * n
+    rhie_chow = (4 * (n + 1) + 4 * L + 48 * F + 32 * n + 8 * n + 24 * n + 8 * F + 4 * Fb + 24 * n + 24 * n
+                 + 8 * F)
+    correct = 104 * n
+    return {"assembly": assembly, "hbya": hbya, "rhie_chow": rhie_chow, "correct": correct}
+
+
+def momentum_measure(s, hip, lib, reps, log):
+    """The momentum side of the SIMPLE corrector on the benchmark mesh, device
+    pointers throughout (lid-driven boundary data, laminar viscosity, a
+    random velocity and pressure): per-call seconds of each entry point and
+    the achieved rate against momentum_bytes."""
+    import ctypes as C
+
+    fv = s.extra["fv"]
+    m = s.mesh
+    nc, nf = m.n_cells, m.n_faces
+    rng = np.random.default_rng(11)
+    vt = np.ones(m.n_patches, np.uint8)
+    vv = np.zeros(3 * m.n_patches)
+    vv[9:12] = [1.0, 0.0, 0.0]
+    pt = np.zeros(m.n_patches, np.uint8)
+    pv = np.zeros(m.n_patches)
+    U = 0.1 * rng.standard_normal(3 * nc)
+    p = rng.standard_normal(nc)
+    phi = fv.mass_flux(vt, vv, 1.0, U)
+    gp = fv.gauss_gradient(pt, pv, p)
+    mu = np.full(nc, 1e-3)
+    rp, cols = fv.momentum_pattern()
+    nnz = len(cols)
+    bufs = {}
+
+    def dev(name, nbytes, host=None):
+        d = C.c_void_p()
+        assert hip.hipMalloc(C.byref(d), C.c_size_t(nbytes)) == 0
+        if host is not None:
+            assert hip.hipMemcpy(d, host.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1) == 0
+        bufs[name] = d
+        return d
+
+    try:
+        d_mu, d_phi, d_gp, d_U, d_p = (dev("mu", 8 * nc, mu), dev("phi", 8 * nf, phi), dev("gp", 24 * nc, gp),
+                                       dev("U", 24 * nc, U), dev("p", 8 * nc, p))
+        d_vals, d_rhs, d_diag = dev("vals", 8 * nnz), dev("rhs", 24 * nc), dev("diag", 8 * nc)
+        d_rau, d_hbya, d_phi2, d_U2 = dev("rau", 24 * nc), dev("hbya", 24 * nc), dev("phi2", 8 * nf), dev("U2", 24 * nc)
+        h = fv._h
+
+        def asm():
+            return lib.cfdxhip_fv_momentum_assemble_device(h, vt, vv, d_mu, d_phi, d_gp, None, d_U, 1, 0.7, d_vals,
+                                                           d_rhs, d_diag)
+
+        def hb():
+            return lib.cfdxhip_fv_momentum_hbya_device(h, d_vals, d_rhs, d_diag, d_gp, d_U, d_rau, d_hbya)
+
+        def rc():
+            return lib.cfdxhip_fv_rhie_chow_flux_device(h, vt, vv, pt, pv, 1.0, d_hbya, d_p, d_rau, d_phi2)
+
+        def corr():
+            return lib.cfdxhip_fv_velocity_correct_device(h, d_hbya, d_rau, d_gp, d_U2)
+
+        by = momentum_bytes(m, nnz)
+        out = {}
+        for name, fn in (("assembly", asm), ("hbya", hb), ("rhie_chow", rc), ("correct", corr)):
+            assert fn() == 0, fv.last_error()
+            assert hip.hipDeviceSynchronize() == 0
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                assert fn() == 0, fv.last_error()
+            assert hip.hipDeviceSynchronize() == 0
+            t = (time.perf_counter() - t0) / reps
+            out[name] = {"ms": round(t * 1e3, 3), "bytes": int(by[name]), "TBps": round(by[name] / t / 1e12, 3)}
+        total_t = sum(v["ms"] for v in out.values()) / 1e3
+        total_b = sum(by.values())
+        out["total"] = {"ms": round(total_t * 1e3, 3), "bytes": int(total_b),
+                        "TBps": round(total_b / total_t / 1e12, 3)}
+        log(f"[bench] momentum side: {out}")
+        return out
+    finally:
+        for d in bufs.values():
+            hip.hipFree(d)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +221,8 @@ def main():
     ap.add_argument("--train-n", type=int, default=48, help="training boxes are train_n^3")
     ap.add_argument("--train-systems", type=int, default=12)
     ap.add_argument("--train-steps", type=int, default=120)
+    ap.add_argument("--no-momentum", action="store_true", help="skip the momentum-side measurement")
+    ap.add_argument("--momentum-repeat", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -293,6 +388,10 @@ def main():
     gbs32 = b32 / t32 / 1e9
     log(f"[bench] SpMV FP64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s | FP32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s")
 
+    momentum = None
+    if not args.no_momentum:
+        momentum = momentum_measure(s, hip, lib, args.momentum_repeat, log)
+
     cpu = None
     if not args.no_cpu:
         t0 = time.perf_counter()
@@ -361,6 +460,10 @@ def main():
         "spmv_fp64_frac_of_peak": round(gbs64 / HBM_PEAK_GBS, 4),
         "device_copy_TBps": round(copy_gbs / 1e3, 4),
         "level_kernels": level_rates,
+        "momentum_side": None if momentum is None else {
+            "what": "three relaxed momentum equations (UPWIND, WLS non-orthogonal correction, alpha_u 0.7), "
+                    "rAU/HbyA, Rhie-Chow flux, velocity correction; device pointers, per call",
+            **momentum},
         "roofline": {
             "bound": "hbm",
             "achieved": round(gbs32, 1),

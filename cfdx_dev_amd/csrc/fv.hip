@@ -518,10 +518,11 @@ __global__ __launch_bounds__(kBlock) void wls_gradient_kernel(
     uint32_t npatch, const uint8_t* __restrict__ kind, const uint8_t* __restrict__ bct,
     const double* __restrict__ bcv, uint32_t bstride, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
     int dim, double limit, const double* __restrict__ field, double* __restrict__ grad,
-    uint8_t* __restrict__ fallback, int* __restrict__ err) {
+    uint8_t* __restrict__ fallback, const int* __restrict__ gate, int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
+    if (gate && !(gate[0] && gate[1])) return;  // momentum path: only when a face is non-orthogonal
     const size_t n1 = nc, n3 = 3 * size_t(nc);
     const uint32_t k0 = coff[c], k1 = coff[c + 1];
     const D3 centre = cc.get(c);
@@ -819,11 +820,12 @@ __global__ __launch_bounds__(kBlock) void momentum_face_kernel(
     const uint32_t* __restrict__ patch, uint32_t npatch, const uint8_t* __restrict__ kind,
     const uint8_t* __restrict__ vbt, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
     const double* __restrict__ mu, const double* __restrict__ phi, const double* __restrict__ ugrad, int bounded,
-    int needs, double* __restrict__ aon, double* __restrict__ ano, double* __restrict__ nof,
+    const int* __restrict__ flags, double* __restrict__ aon, double* __restrict__ ano, double* __restrict__ nof,
     int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= nf) return;
+    const bool needs = flags[0] && flags[1];
     const size_t n1 = nc, n3 = 3 * size_t(nc);
     const uint32_t o = owner[f];
     const double F = phi[f];
@@ -925,11 +927,12 @@ __global__ __launch_bounds__(kBlock) void momentum_cell_kernel(
     const double* __restrict__ vbv, const double* __restrict__ vol, const double* __restrict__ phi,
     const double* __restrict__ aon, const double* __restrict__ ano, const double* __restrict__ nof,
     const double* __restrict__ gradp, const double* __restrict__ body, const double* __restrict__ U, int bounded,
-    int needs, int relax, double alpha, double* __restrict__ diag_out, double* __restrict__ rhs_out,
-    int* __restrict__ err) {
+    const int* __restrict__ flags, int relax, double alpha, double* __restrict__ diag_out,
+    double* __restrict__ rhs_out, int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
+    const bool needs = flags[0] && flags[1];
     const size_t n1 = nc;
     double diag = 0.0, div = 0.0;
     double rhs[3] = {0.0, 0.0, 0.0}, def[3] = {0.0, 0.0, 0.0};
@@ -1148,7 +1151,7 @@ FvMesh::~FvMesh() { release(); }
 void FvMesh::release() {
     for (double** p : {&px, &py, &pz, &fcx, &fcy, &fcz, &fsx, &fsy, &fsz, &ccx, &ccy, &ccz, &vol, &work_coeff,
                        &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi, &mom_aon,
-                       &mom_ano, &mom_nof, &mom_ugrad})
+                       &mom_ano, &mom_nof, &mom_ugrad, &bc_f64})
         dfree(*p);
     for (uint32_t** p : {&foff, &fvtx, &owner, &coff, &cfaces, &cfaces_sorted, &patch})
         dfree(*p);
@@ -1156,6 +1159,7 @@ void FvMesh::release() {
     dfree(patch_kind);
     dfree(err);
     dfree(mom_flags);
+    dfree(bc_u8);
     for (auto* pl : {&pressure_full, &pressure_red})
         for (uint32_t** p : {&pl->rowp, &pl->cols, &pl->nz_row_diag, &pl->nzf_off, &pl->nzf, &pl->red_src,
                              &pl->red_rhs_src})
@@ -1441,16 +1445,35 @@ const PressurePlan& FvMesh::diffusion_assemble_device(double gamma, const double
 
 // ============================================================ momentum side
 
-void FvMesh::ensure_momentum_work(bool with_gradient) {
-    if (!mom_aon) {
-        mom_aon = dalloc<double>(nf);
-        mom_ano = dalloc<double>(nf);
-        mom_flags = dalloc<int>(4);
+void FvMesh::ensure_momentum_work() {
+    if (mom_aon) return;
+    mom_aon = dalloc<double>(nf);
+    mom_ano = dalloc<double>(nf);
+    mom_nof = dalloc<double>(3 * uint64_t(nf));
+    mom_ugrad = dalloc<double>(9 * uint64_t(nc));
+    mom_flags = dalloc<int>(4);
+    const uint32_t npp = std::max<uint32_t>(npatch, 1);
+    bc_u8 = dalloc<uint8_t>(2 * uint64_t(npp));
+    bc_f64 = dalloc<double>(4 * uint64_t(npp));
+}
+
+// Boundary data per patch into the persistent device buffers (no allocation
+// per call): bc_u8 = [velocity type | pressure type], bc_f64 = [3 velocity
+// values | pressure value]. Types other than 1 read as zero-gradient.
+void FvMesh::upload_bcs(const uint8_t* h_vbt, const double* h_vbv, const uint8_t* h_pbt, const double* h_pbv) {
+    ensure_momentum_work();
+    const uint32_t npp = std::max<uint32_t>(npatch, 1);
+    h_bc_u8.assign(2 * size_t(npp), 0);
+    h_bc_f64.assign(4 * size_t(npp), 0.0);
+    for (uint32_t p = 0; p < npatch; ++p) {
+        if (h_vbt) h_bc_u8[p] = h_vbt[p] == 1 ? 1 : 0;
+        if (h_pbt) h_bc_u8[npp + p] = h_pbt[p] == 1 ? 1 : 0;
+        for (int k = 0; k < 3; ++k)
+            if (h_vbv) h_bc_f64[3 * size_t(p) + k] = h_vbv[3 * size_t(p) + k];
+        if (h_pbv) h_bc_f64[3 * size_t(npp) + p] = h_pbv[p];
     }
-    if (with_gradient && !mom_nof) {
-        mom_nof = dalloc<double>(3 * uint64_t(nf));
-        mom_ugrad = dalloc<double>(9 * uint64_t(nc));
-    }
+    HIP_OK(hipMemcpyAsync(bc_u8, h_bc_u8.data(), h_bc_u8.size(), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemcpyAsync(bc_f64, h_bc_f64.data(), h_bc_f64.size() * 8, hipMemcpyHostToDevice, stream));
 }
 
 int FvMesh::stencil_dimension() const {
@@ -1469,71 +1492,48 @@ void FvMesh::wls_gradient_device(const uint8_t* h_bt, const double* h_bv, double
     wls_gradient_kernel<1><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
                                                                 patch_kind, d_bt, d_bv, 1, fc, fs, cc, vol,
                                                                 stencil_dimension(), limit, d_field, d_grad,
-                                                                d_fallback, err);
+                                                                d_fallback, nullptr, err);
     HIP_OK(hipStreamSynchronize(stream));
     dfree(d_bt);
     dfree(d_bv);
     check_err("boundary WLS gradient: invalid geometry, boundary data or stencil size");
 }
 
+// One stream-ordered pass, synchronised once at the end for the error flag:
+// viscosity check and the non-orthogonality scan set device flags that the
+// gradient, face and cell kernels read (the WLS velocity gradients are only
+// computed when a face is non-orthogonal and diffusion is active).
 const PressurePlan& FvMesh::momentum_assemble_device(const uint8_t* h_vbt, const double* h_vbv, const double* d_mu,
                                                      const double* d_phi, const double* d_gradp,
                                                      const double* d_body, const double* d_U, bool bounded,
                                                      double alpha, double* d_vals, double* d_rhs, double* d_diag) {
     if (!(alpha > 0.0 && alpha <= 1.0)) throw std::invalid_argument("relax_momentum_equation: invalid relaxation");
     const PressurePlan& plan = pressure_plan(false, 0);
-    ensure_momentum_work(false);
-    const uint32_t npp = std::max<uint32_t>(npatch, 1);
-    std::vector<uint8_t> vt(npp, 0);
-    std::vector<double> vv(3 * size_t(npp), 0.0);
-    for (uint32_t p = 0; p < npatch; ++p) {
-        vt[p] = h_vbt[p] == 1 ? 1 : 0;
-        for (int k = 0; k < 3; ++k) vv[3 * size_t(p) + k] = h_vbv[3 * size_t(p) + k];
-    }
-    uint8_t* d_vbt = upload(vt.data(), npp);
-    double* d_vbv = upload(vv.data(), vv.size());
+    upload_bcs(h_vbt, h_vbv, nullptr, nullptr);
+    const uint8_t* d_vbt = bc_u8;
+    const double* d_vbv = bc_f64;
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     HIP_OK(hipMemsetAsync(mom_flags, 0, 4 * sizeof(int), stream));
     viscosity_check_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, d_mu, mom_flags, err);
-    int flags[4] = {0, 0, 0, 0};
-    HIP_OK(hipMemcpyAsync(flags, mom_flags, sizeof flags, hipMemcpyDeviceToHost, stream));
+    nonorth_scan_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, patch, npatch, patch_kind, d_vbt, fc,
+                                                             fs, cc, mom_flags, err);
+    wls_gradient_kernel<3><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
+                                                                patch_kind, d_vbt, d_vbv, 3, fc, fs, cc, vol,
+                                                                stencil_dimension(), 1.e8, d_U, mom_ugrad, nullptr,
+                                                                mom_flags, err);
+    momentum_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind, d_vbt,
+                                                              fc, fs, cc, vol, d_mu, d_phi, mom_ugrad,
+                                                              bounded ? 1 : 0, mom_flags, mom_aon, mom_ano, mom_nof,
+                                                              err);
+    momentum_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(
+        nc, nf, coff, cfaces_sorted ? cfaces_sorted : cfaces, owner, neigh, patch, npatch, patch_kind, d_vbt, d_vbv,
+        vol, d_phi, mom_aon, mom_ano, mom_nof, d_gradp, d_body, d_U, bounded ? 1 : 0, mom_flags,
+        alpha != 1.0 ? 1 : 0, alpha, d_diag, d_rhs, err);
+    momentum_values_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, plan.nzf_off, plan.nzf,
+                                                                owner, d_diag, mom_aon, mom_ano, d_vals);
     HIP_OK(hipStreamSynchronize(stream));
-    try {
-        check_err("assemble_momentum_component: cell viscosity must be finite and non-negative");
-        if (flags[0]) {
-            nonorth_scan_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, patch, npatch, patch_kind,
-                                                                     d_vbt, fc, fs, cc, mom_flags, err);
-            HIP_OK(hipMemcpyAsync(flags, mom_flags, sizeof flags, hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipStreamSynchronize(stream));
-            check_err("assemble_scalar_equation: invalid diffusion decomposition");
-        }
-        const bool needs = flags[0] && flags[1];
-        if (needs) {
-            ensure_momentum_work(true);
-            wls_gradient_kernel<3><<<grid_for(nc), kBlock, 0, stream>>>(
-                nc, coff, cfaces, owner, neigh, patch, npatch, patch_kind, d_vbt, d_vbv, 3, fc, fs, cc, vol,
-                stencil_dimension(), 1.e8, d_U, mom_ugrad, nullptr, err);
-        }
-        momentum_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind,
-                                                                  d_vbt, fc, fs, cc, vol, d_mu, d_phi, mom_ugrad,
-                                                                  bounded ? 1 : 0, needs ? 1 : 0, mom_aon, mom_ano,
-                                                                  mom_nof, err);
-        momentum_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(
-            nc, nf, coff, cfaces_sorted ? cfaces_sorted : cfaces, owner, neigh, patch, npatch, patch_kind, d_vbt,
-            d_vbv, vol, d_phi, mom_aon, mom_ano, mom_nof, d_gradp, d_body, d_U, bounded ? 1 : 0, needs ? 1 : 0,
-            alpha != 1.0 ? 1 : 0, alpha, d_diag, d_rhs, err);
-        momentum_values_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, plan.nzf_off, plan.nzf,
-                                                                    owner, d_diag, mom_aon, mom_ano, d_vals);
-        HIP_OK(hipStreamSynchronize(stream));
-        check_err("assemble_momentum_component: invalid flux, geometry, gradient or non-positive diagonal");
-    } catch (...) {
-        dfree(d_vbt);
-        dfree(d_vbv);
-        throw;
-    }
-    dfree(d_vbt);
-    dfree(d_vbv);
+    check_err("assemble_momentum_component: invalid viscosity, flux or geometry, or non-positive diagonal");
     return plan;
 }
 
@@ -1557,19 +1557,12 @@ void FvMesh::rhie_chow_flux_device(const uint8_t* h_vbt, const double* h_vbv, co
                                    const double* h_pbv, double rho, const double* d_hbya, const double* d_p,
                                    const double* d_rau, double* d_phi) {
     ensure_work();
+    upload_bcs(h_vbt, h_vbv, h_pbt, h_pbv);
     const uint32_t npp = std::max<uint32_t>(npatch, 1);
-    std::vector<uint8_t> vt(npp, 0), pt(npp, 0);
-    std::vector<double> vv(3 * size_t(npp), 0.0), pv(npp, 0.0);
-    for (uint32_t p = 0; p < npatch; ++p) {
-        vt[p] = h_vbt[p];
-        pt[p] = h_pbt[p];
-        pv[p] = h_pbv[p];
-        for (int k = 0; k < 3; ++k) vv[3 * size_t(p) + k] = h_vbv[3 * size_t(p) + k];
-    }
-    uint8_t* d_vbt = upload(vt.data(), npp);
-    uint8_t* d_pbt = upload(pt.data(), npp);
-    double* d_vbv = upload(vv.data(), vv.size());
-    double* d_pbv = upload(pv.data(), npp);
+    const uint8_t* d_vbt = bc_u8;
+    const uint8_t* d_pbt = bc_u8 + npp;
+    const double* d_vbv = bc_f64;
+    const double* d_pbv = bc_f64 + 3 * size_t(npp);
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     positive_check_kernel<<<grid_for(3 * uint64_t(nc)), kBlock, 0, stream>>>(3 * uint64_t(nc), d_rau, 53, err);
@@ -1578,10 +1571,6 @@ void FvMesh::rhie_chow_flux_device(const uint8_t* h_vbt, const double* h_vbv, co
                                                                d_vbt, d_vbv, d_pbt, d_pbv, rho, fc, fs, cc, vol,
                                                                d_hbya, d_p, work_grad, d_rau, d_phi, err);
     HIP_OK(hipStreamSynchronize(stream));
-    dfree(d_vbt);
-    dfree(d_pbt);
-    dfree(d_vbv);
-    dfree(d_pbv);
     check_err("make_rhie_chow_mass_flux: inverse momentum diagonal must be finite and positive, or invalid face");
 }
 

@@ -51,6 +51,11 @@ struct FvMesh {
     // fluxes (3 x nf), velocity WLS gradients (3 x 3 x nc), flags
     double *mom_aon = nullptr, *mom_ano = nullptr, *mom_nof = nullptr, *mom_ugrad = nullptr;
     int* mom_flags = nullptr;
+    // per-patch boundary data of the momentum-side calls (persistent)
+    uint8_t* bc_u8 = nullptr;
+    double* bc_f64 = nullptr;
+    std::vector<uint8_t> h_bc_u8;
+    std::vector<double> h_bc_f64;
     PressurePlan pressure_full, pressure_red;
     // host copies of the addressing
     std::vector<uint32_t> h_owner, h_coff, h_cf, h_patch;
@@ -91,7 +96,8 @@ struct FvMesh {
     void rhie_chow_flux_device(const uint8_t* h_vbt, const double* h_vbv, const uint8_t* h_pbt,
                                const double* h_pbv, double rho, const double* d_hbya, const double* d_p,
                                const double* d_rau, double* d_phi);
-    void ensure_momentum_work(bool with_gradient);
+    void ensure_momentum_work();
+    void upload_bcs(const uint8_t* h_vbt, const double* h_vbv, const uint8_t* h_pbt, const double* h_pbv);
     int stencil_dimension() const;
     void ensure_work();
     void check_err(const char* what);
