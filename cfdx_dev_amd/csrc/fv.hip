@@ -26,6 +26,7 @@
 #include <numeric>
 #include <stdexcept>
 
+#include "amg_setup.h"
 #include "kernels.hpp"
 #include "parallel.h"
 
@@ -451,12 +452,22 @@ __global__ __launch_bounds__(kBlock) void diffusion_cell_kernel(uint32_t nc, con
     rhs[c] = b;
 }
 
-__global__ __launch_bounds__(kBlock) void soa_to_aos_kernel(uint32_t n, Soa3 s, double* __restrict__ aos) {
+// Caller-ordered AoS copy of an internally ordered SoA triple: entry i of the
+// caller's numbering is internal entry map[i].
+__global__ __launch_bounds__(kBlock) void soa_to_aos_kernel(uint32_t n, const uint32_t* __restrict__ map, Soa3 s,
+                                                           double* __restrict__ aos) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    aos[3 * size_t(i)] = s.x[i];
-    aos[3 * size_t(i) + 1] = s.y[i];
-    aos[3 * size_t(i) + 2] = s.z[i];
+    const uint32_t j = map[i];
+    aos[3 * size_t(i)] = s.x[j];
+    aos[3 * size_t(i) + 1] = s.y[j];
+    aos[3 * size_t(i) + 2] = s.z[j];
+}
+
+__global__ __launch_bounds__(kBlock) void gather_u8_kernel(uint64_t n, const uint32_t* __restrict__ src_idx,
+                                                          const uint8_t* __restrict__ src, uint8_t* __restrict__ dst) {
+    const uint64_t k = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (k < n) dst[k] = src[src_idx[k]];
 }
 
 // ------------------------------------------------------ momentum side ---
@@ -990,11 +1001,14 @@ __global__ __launch_bounds__(kBlock) void momentum_cell_kernel(
     for (int k = 0; k < 3; ++k) rhs_out[k * n1 + c] = rhs[k];
 }
 
-// CSR values of the momentum matrix, one thread per row: the diagonal from
-// diag; off-diagonal (r, j) = 0 - the coefficient of each face joining r and j
-// in ascending face order (ano when r owns the face, else aon).
+// CSR values of the momentum matrix (caller numbering), one thread per row:
+// the diagonal from diag; off-diagonal (r, j) = 0 - the coefficient of each
+// face joining r and j in ascending caller face order (ano when r owns the
+// face, else aon). rowmap: internal id of each caller row; nzf: internal face
+// ids; owner, diag, aon, ano: internal numbering.
 __global__ __launch_bounds__(kBlock) void momentum_values_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
                                                                 const uint32_t* __restrict__ cols,
+                                                                const uint32_t* __restrict__ rowmap,
                                                                 const uint32_t* __restrict__ nzf_off,
                                                                 const uint32_t* __restrict__ nzf,
                                                                 const uint32_t* __restrict__ owner,
@@ -1005,15 +1019,16 @@ __global__ __launch_bounds__(kBlock) void momentum_values_kernel(uint32_t n, con
 #pragma clang fp contract(off)
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= n) return;
+    const uint32_t ri = rowmap[r];
     for (uint32_t k = rowp[r]; k < rowp[r + 1]; ++k) {
         if (cols[k] == r) {
-            vals[k] = diag[r];
+            vals[k] = diag[ri];
             continue;
         }
         double v = 0.0;
         for (uint32_t q = nzf_off[k]; q < nzf_off[k + 1]; ++q) {
             const uint32_t f = nzf[q];
-            v -= owner[f] == r ? ano[f] : aon[f];
+            v -= owner[f] == ri ? ano[f] : aon[f];
         }
         vals[k] = v;
     }
@@ -1146,15 +1161,30 @@ __global__ __launch_bounds__(kBlock) void rhie_chow_flux_kernel(
 
 // ==================================================================== FvMesh
 
+// Internal numbering. A caller's cells may be numbered without locality (the
+// benchmark boxes are a random permutation), and a kernel that gathers cell or
+// face data through such ids reads a whole cache line per 8-byte value. The
+// handle therefore keeps the mesh in its own order: cells in reverse
+// Cuthill-McKee order of the cell graph, faces grouped by owner in that order
+// (caller order within an owner). Every kernel runs on the internal arrays;
+// the public methods take and return caller-ordered fields and permute them
+// once on the way in and out. Each cell keeps its caller face list order
+// (cfaces) and visits faces in ascending caller face id where the reference
+// accumulates in face order (cfaces_sorted), so every value is unchanged.
+
 FvMesh::~FvMesh() { release(); }
 
 void FvMesh::release() {
-    for (double** p : {&px, &py, &pz, &fcx, &fcy, &fcz, &fsx, &fsy, &fsz, &ccx, &ccy, &ccz, &vol, &work_coeff,
-                       &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi, &mom_aon,
-                       &mom_ano, &mom_nof, &mom_ugrad, &bc_f64})
+    for (double** p : {&px, &py, &pz, &fcx, &fcy, &fcz, &fsx, &fsy, &fsz, &ccx, &ccy, &ccz, &vol, &vol_caller,
+                       &work_coeff, &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi,
+                       &mom_aon, &mom_ano, &mom_nof, &mom_ugrad, &bc_f64})
         dfree(*p);
-    for (uint32_t** p : {&foff, &fvtx, &owner, &coff, &cfaces, &cfaces_sorted, &patch})
+    for (uint32_t** p : {&foff, &fvtx, &owner, &coff, &cfaces, &cfaces_sorted, &patch, &cell_new, &cell_old,
+                         &face_new, &face_old})
         dfree(*p);
+    for (auto& b : scratch) dfree(b.first);
+    scratch.clear();
+    dfree(scratch_u8);
     dfree(neigh);
     dfree(patch_kind);
     dfree(err);
@@ -1178,7 +1208,7 @@ void FvMesh::create(int device, uint64_t np_, const double* hx, const double* hy
     nf = uint32_t(nf_);
     nc = uint32_t(nc_);
     npatch = uint32_t(npatch_);
-    // Host copies needed to build sparsity plans.
+    // Host copies (caller numbering) for the sparsity plans.
     h_owner.assign(hown, hown + nf);
     h_neigh.assign(hnei, hnei + nf);
     h_coff.assign(hcoff, hcoff + nc + 1);
@@ -1190,35 +1220,132 @@ void FvMesh::create(int device, uint64_t np_, const double* hx, const double* hy
     }
     for (uint32_t k = 0; k < h_coff[nc]; ++k)
         if (h_cf[k] >= nf) throw std::invalid_argument("cell face index out of range");
-    // Face-id-ascending copy of each cell's face list (global face order).
-    std::vector<uint32_t> sorted(h_cf);
-    std::vector<uint8_t> unsorted(nc, 0);
-    parallel_for(nc, 0, [&](uint64_t c) {
-        std::sort(sorted.begin() + h_coff[c], sorted.begin() + h_coff[c + 1]);
-        for (uint32_t k = h_coff[c]; k < h_coff[c + 1]; ++k)
-            if (sorted[k] != h_cf[k]) unsorted[c] = 1;
-    });
-    bool already = true;
-    for (uint8_t u : unsorted) already = already && !u;
+    for (uint32_t k = 0; k < hfoff[nf]; ++k)
+        if (hfvtx[k] >= np) throw std::invalid_argument("face vertex out of range");
+    // Cell order: reverse Cuthill-McKee of the cell graph.
+    std::vector<uint32_t> cnew, cold(nc), fnew(nf), fold(nf);
+    {
+        HCsr g;
+        g.n = g.ncols = nc;
+        g.rowp.assign(uint64_t(nc) + 1, 0);
+        for (uint32_t c = 0; c < nc; ++c) {
+            uint32_t d = 0;
+            for (uint32_t k = h_coff[c]; k < h_coff[c + 1]; ++k) d += h_neigh[h_cf[k]] >= 0;
+            g.rowp[c + 1] = g.rowp[c] + d;
+        }
+        g.cols.resize(g.rowp[nc]);
+        parallel_for(nc, 0, [&](uint64_t c) {
+            uint32_t w = g.rowp[c];
+            for (uint32_t k = h_coff[c]; k < h_coff[c + 1]; ++k) {
+                const uint32_t f = h_cf[k];
+                if (h_neigh[f] < 0) continue;
+                g.cols[w++] = h_owner[f] == c ? uint32_t(h_neigh[f]) : h_owner[f];
+            }
+        });
+        cnew = rcm_ordering(g);
+    }
+    for (uint32_t c = 0; c < nc; ++c) cold[cnew[c]] = c;
+    // Face order: by internal owner, caller order within an owner.
+    {
+        std::vector<uint32_t> cnt(uint64_t(nc) + 1, 0);
+        for (uint32_t f = 0; f < nf; ++f) ++cnt[cnew[h_owner[f]] + 1];
+        for (uint32_t c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint32_t pos = cnt[cnew[h_owner[f]]]++;
+            fold[pos] = f;
+            fnew[f] = pos;
+        }
+    }
+    // Internal arrays.
+    {
+        std::vector<uint32_t> o(nf), pt(nf), fo(uint64_t(nf) + 1, 0), fv;
+        std::vector<int32_t> nb(nf);
+        for (uint32_t i = 0; i < nf; ++i) {
+            const uint32_t f = fold[i];
+            o[i] = cnew[h_owner[f]];
+            nb[i] = h_neigh[f] >= 0 ? int32_t(cnew[uint32_t(h_neigh[f])]) : -1;
+            pt[i] = h_patch[f];
+            fo[i + 1] = fo[i] + (hfoff[f + 1] - hfoff[f]);
+        }
+        fv.resize(fo[nf]);
+        parallel_for(nf, 0, [&](uint64_t i) {
+            const uint32_t f = fold[i];
+            std::copy(hfvtx + hfoff[f], hfvtx + hfoff[f + 1], fv.begin() + fo[i]);
+        });
+        owner = upload(o.data(), nf);
+        neigh = upload(nb.data(), nf);
+        patch = upload(pt.data(), nf);
+        foff = upload(fo.data(), uint64_t(nf) + 1);
+        fvtx = upload(fv.data(), fv.size());
+    }
+    {
+        std::vector<uint32_t> co(uint64_t(nc) + 1, 0), cf(h_cf.size()), cfs(h_cf.size());
+        for (uint32_t i = 0; i < nc; ++i) {
+            const uint32_t c = cold[i];
+            co[i + 1] = co[i] + (h_coff[c + 1] - h_coff[c]);
+        }
+        parallel_for(nc, 0, [&](uint64_t i) {
+            const uint32_t c = cold[i];
+            const uint32_t b = h_coff[c], e = h_coff[c + 1];
+            std::vector<uint32_t> sorted(h_cf.begin() + b, h_cf.begin() + e);
+            std::sort(sorted.begin(), sorted.end());
+            for (uint32_t k = 0; k < e - b; ++k) {
+                cf[co[i] + k] = fnew[h_cf[b + k]];
+                cfs[co[i] + k] = fnew[sorted[k]];
+            }
+        });
+        coff = upload(co.data(), co.size());
+        cfaces = upload(cf.data(), cf.size());
+        cfaces_sorted = upload(cfs.data(), cfs.size());
+    }
+    cell_new = upload(cnew.data(), nc);
+    cell_old = upload(cold.data(), nc);
+    face_new = upload(fnew.data(), nf);
+    face_old = upload(fold.data(), nf);
+    h_cell_new = std::move(cnew);
+    h_face_new = std::move(fnew);
     px = upload(hx, np);
     py = upload(hy, np);
     pz = upload(hz, np);
-    foff = upload(hfoff, uint64_t(nf) + 1);
-    fvtx = upload(hfvtx, uint64_t(hfoff[nf]));
-    for (uint32_t k = 0; k < hfoff[nf]; ++k)
-        if (hfvtx[k] >= np) throw std::invalid_argument("face vertex out of range");
-    owner = upload(hown, nf);
-    neigh = upload(hnei, nf);
-    coff = upload(hcoff, uint64_t(nc) + 1);
-    cfaces = upload(hcf, uint64_t(h_coff[nc]));
-    cfaces_sorted = already ? nullptr : upload(sorted.data(), sorted.size());
-    patch = upload(hpatch, nf);
     patch_kind = upload(hkind, npatch);
     for (double** p : {&fcx, &fcy, &fcz, &fsx, &fsy, &fsz}) *p = dalloc<double>(nf);
-    for (double** p : {&ccx, &ccy, &ccz, &vol}) *p = dalloc<double>(nc);
+    for (double** p : {&ccx, &ccy, &ccz, &vol, &vol_caller}) *p = dalloc<double>(nc);
     err = dalloc<int>(4);
     HIP_OK(hipStreamCreate(&stream));
     compute_geometry();
+    gather_u32idx_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, cell_new, vol, vol_caller);
+    HIP_OK(hipStreamSynchronize(stream));
+}
+
+double* FvMesh::scr(size_t slot, uint64_t count) {
+    if (scratch.size() <= slot) scratch.resize(slot + 1, {nullptr, 0});
+    auto& b = scratch[slot];
+    if (b.second < count) {
+        dfree(b.first);
+        b.first = dalloc<double>(count);
+        b.second = count;
+    }
+    return b.first;
+}
+
+void FvMesh::cells_in(const double* caller, double* internal, int comps) {
+    for (int k = 0; k < comps; ++k)
+        gather_u32idx_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, cell_old, caller + size_t(k) * nc,
+                                                                  internal + size_t(k) * nc);
+}
+
+void FvMesh::cells_out(const double* internal, double* caller, int comps) {
+    for (int k = 0; k < comps; ++k)
+        gather_u32idx_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, cell_new, internal + size_t(k) * nc,
+                                                                  caller + size_t(k) * nc);
+}
+
+void FvMesh::faces_in(const double* caller, double* internal) {
+    gather_u32idx_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, face_old, caller, internal);
+}
+
+void FvMesh::faces_out(const double* internal, double* caller) {
+    gather_u32idx_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, face_new, internal, caller);
 }
 
 void FvMesh::check_err(const char* what) {
@@ -1243,16 +1370,16 @@ void FvMesh::compute_geometry() {
 
 void FvMesh::geometry_out(double* fc_aos, double* fs_aos, double* cc_aos, double* vol_out) {
     double* tmp = dalloc<double>(3 * uint64_t(std::max(nf, nc)));
-    auto out3 = [&](Soa3 s, uint32_t n, double* h) {
+    auto out3 = [&](Soa3 s, uint32_t n, const uint32_t* map, double* h) {
         if (!h) return;
-        soa_to_aos_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, s, tmp);
+        soa_to_aos_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, map, s, tmp);
         HIP_OK(hipStreamSynchronize(stream));
         download(h, tmp, 3 * uint64_t(n));
     };
-    out3(Soa3{fcx, fcy, fcz}, nf, fc_aos);
-    out3(Soa3{fsx, fsy, fsz}, nf, fs_aos);
-    out3(Soa3{ccx, ccy, ccz}, nc, cc_aos);
-    download(vol_out, vol, nc);
+    out3(Soa3{fcx, fcy, fcz}, nf, face_new, fc_aos);
+    out3(Soa3{fsx, fsy, fsz}, nf, face_new, fs_aos);
+    out3(Soa3{ccx, ccy, ccz}, nc, cell_new, cc_aos);
+    download(vol_out, vol_caller, nc);
     dfree(tmp);
 }
 
@@ -1268,18 +1395,30 @@ void FvMesh::ensure_work() {
     fixed_face = dalloc<uint8_t>(nf);
 }
 
-void FvMesh::gauss_gradient_device(const uint8_t* d_bct, const double* d_bcv, const double* d_p, double* d_grad) {
+void FvMesh::gauss_gradient_internal(const uint8_t* d_bct, const double* d_bcv, const double* p_i, double* grad_i) {
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     gauss_gradient_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
-                                                               patch_kind, d_bct, d_bcv, fc, fs, cc, vol, d_p,
-                                                               d_grad);
+                                                               patch_kind, d_bct, d_bcv, fc, fs, cc, vol, p_i,
+                                                               grad_i);
+}
+
+void FvMesh::gauss_gradient_device(const uint8_t* d_bct, const double* d_bcv, const double* d_p, double* d_grad) {
+    double* p_i = scr(9, nc);
+    double* g_i = scr(8, 3 * uint64_t(nc));
+    cells_in(d_p, p_i, 1);
+    gauss_gradient_internal(d_bct, d_bcv, p_i, g_i);
+    cells_out(g_i, d_grad, 3);
 }
 
 void FvMesh::mass_flux_device(const uint8_t* d_vbt, const double* d_vbv, double rho, const double* d_U,
                               double* d_phi) {
     const Soa3 fs{fsx, fsy, fsz};
+    double* u_i = scr(0, 3 * uint64_t(nc));
+    double* phi_i = scr(4, nf);
+    cells_in(d_U, u_i, 3);
     mass_flux_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind, d_vbt,
-                                                          d_vbv, rho, fs, d_U, d_phi);
+                                                          d_vbv, rho, fs, u_i, phi_i);
+    faces_out(phi_i, d_phi);
 }
 
 // Sparsity of the pressure system: diagonal + face neighbours, sorted per
@@ -1339,6 +1478,12 @@ const PressurePlan& FvMesh::pressure_plan(bool eliminate, uint32_t ref) {
                 }
             }
         });
+        // Device slot maps address the internal numbering: the diagonal's
+        // internal row and the joining faces' internal ids (still listed in
+        // ascending caller face id).
+        for (auto& v : rdiag)
+            if (v != 0xffffffffu) v = h_cell_new[v];
+        for (auto& f : nzf) f = h_face_new[f];
         full.n = nc;
         full.nnz = nnz;
         full.h_rowp = std::move(rowp);
@@ -1356,7 +1501,7 @@ const PressurePlan& FvMesh::pressure_plan(bool eliminate, uint32_t ref) {
         std::vector<uint32_t> rowp{0}, cols, src, rsrc;
         for (uint32_t c = 0; c < nc; ++c) {
             if (c == ref) continue;
-            rsrc.push_back(c);
+            rsrc.push_back(h_cell_new[c]);
             for (uint32_t k = full.h_rowp[c]; k < full.h_rowp[c + 1]; ++k) {
                 const uint32_t j = full.h_cols[k];
                 if (j == ref) continue;
@@ -1394,22 +1539,29 @@ const PressurePlan& FvMesh::pressure_assemble_device(const uint8_t* h_pbt, const
     uint8_t* d_pbt = upload(h_pbt, npatch);
     double* d_pbv = upload(h_pbv, npatch);
     std::vector<uint8_t> ff(nf, 0);
-    for (uint32_t f = 0; f < nf; ++f) ff[f] = h_neigh[f] < 0 && h_patch[f] < npatch && h_pbt[h_patch[f]] == 1;
+    for (uint32_t f = 0; f < nf; ++f)
+        ff[h_face_new[f]] = h_neigh[f] < 0 && h_patch[f] < npatch && h_pbt[h_patch[f]] == 1;
     HIP_OK(hipMemcpy(fixed_face, ff.data(), nf, hipMemcpyHostToDevice));
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
-    gauss_gradient_device(d_pbt, d_pbv, d_p, work_grad);
+    double* rau_i = scr(5, 3 * uint64_t(nc));
+    double* phi_i = scr(4, nf);
+    double* p_i = scr(9, nc);
+    cells_in(d_rAU, rau_i, 3);
+    faces_in(d_phi, phi_i);
+    cells_in(d_p, p_i, 1);
+    gauss_gradient_internal(d_pbt, d_pbv, p_i, work_grad);
     pressure_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, d_pbt, d_pbv, rho,
-                                                              fc, fs, cc, vol, d_rAU, d_p, work_grad, work_coeff,
+                                                              fc, fs, cc, vol, rau_i, p_i, work_grad, work_coeff,
                                                               work_rc, work_bsrc, err);
-    pressure_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, cfaces_sorted ? cfaces_sorted : cfaces,
-                                                              owner, neigh, d_phi, work_coeff, work_rc, work_bsrc,
-                                                              fixed_face, work_diag, work_rhs);
+    pressure_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, cfaces_sorted, owner, neigh, phi_i,
+                                                              work_coeff, work_rc, work_bsrc, fixed_face, work_diag,
+                                                              work_rhs);
     if (!eliminate) {
         csr_values_kernel<<<grid_for(plan.nnz), kBlock, 0, stream>>>(plan.nnz, pressure_full.nz_row_diag,
                                                                      pressure_full.nzf_off, pressure_full.nzf,
                                                                      work_diag, work_coeff, d_vals);
-        HIP_OK(hipMemcpyAsync(d_rhs, work_rhs, uint64_t(nc) * 8, hipMemcpyDeviceToDevice, stream));
+        cells_out(work_rhs, d_rhs, 1);
     } else {
         if (!work_vals) work_vals = dalloc<double>(pressure_full.nnz);
         csr_values_kernel<<<grid_for(pressure_full.nnz), kBlock, 0, stream>>>(
@@ -1431,13 +1583,20 @@ const PressurePlan& FvMesh::diffusion_assemble_device(double gamma, const double
     const PressurePlan& plan = pressure_plan(false, 0);
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
-    diffusion_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, gamma, fc, fs, cc, d_fval,
+    double* src_i = scr(3, nc);
+    double* fval_i = scr(7, nf);
+    double* rhs_i = scr(8, nc);
+    if (!scratch_u8) scratch_u8 = dalloc<uint8_t>(nf);
+    cells_in(d_source, src_i, 1);
+    faces_in(d_fval, fval_i);
+    gather_u8_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, face_old, d_fneu, scratch_u8);
+    diffusion_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, gamma, fc, fs, cc, fval_i,
                                                                work_coeff, err);
-    diffusion_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces_sorted ? cfaces_sorted : cfaces,
-                                                               neigh, fs, vol, d_source, d_fval, d_fneu, work_coeff,
-                                                               work_diag, d_rhs);
+    diffusion_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces_sorted, neigh, fs, vol, src_i, fval_i,
+                                                               scratch_u8, work_coeff, work_diag, rhs_i);
     csr_values_kernel<<<grid_for(plan.nnz), kBlock, 0, stream>>>(plan.nnz, plan.nz_row_diag, plan.nzf_off, plan.nzf,
                                                                  work_diag, work_coeff, d_vals);
+    cells_out(rhs_i, d_rhs, 1);
     HIP_OK(hipStreamSynchronize(stream));
     check_err("assemble_cell_diffusion_matrix: invalid geometry");
     return plan;
@@ -1489,10 +1648,14 @@ void FvMesh::wls_gradient_device(const uint8_t* h_bt, const double* h_bv, double
     double* d_bv = upload(h_bv, std::max<uint32_t>(npatch, 1));
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    double* f_i = scr(0, nc);
+    double* g_i = scr(8, 3 * uint64_t(nc));
+    cells_in(d_field, f_i, 1);
     wls_gradient_kernel<1><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
                                                                 patch_kind, d_bt, d_bv, 1, fc, fs, cc, vol,
-                                                                stencil_dimension(), limit, d_field, d_grad,
-                                                                d_fallback, nullptr, err);
+                                                                stencil_dimension(), limit, f_i, g_i, d_fallback,
+                                                                nullptr, err);
+    cells_out(g_i, d_grad, 3);
     HIP_OK(hipStreamSynchronize(stream));
     dfree(d_bt);
     dfree(d_bv);
@@ -1513,25 +1676,39 @@ const PressurePlan& FvMesh::momentum_assemble_device(const uint8_t* h_vbt, const
     const uint8_t* d_vbt = bc_u8;
     const double* d_vbv = bc_f64;
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
+    double* u_i = scr(0, 3 * uint64_t(nc));
+    double* gp_i = scr(1, 3 * uint64_t(nc));
+    double* body_i = d_body ? scr(2, 3 * uint64_t(nc)) : nullptr;
+    double* mu_i = scr(3, nc);
+    double* phi_i = scr(4, nf);
+    double* rhs_i = scr(8, 3 * uint64_t(nc));
+    double* diag_i = scr(9, nc);
+    cells_in(d_U, u_i, 3);
+    cells_in(d_gradp, gp_i, 3);
+    if (d_body) cells_in(d_body, body_i, 3);
+    cells_in(d_mu, mu_i, 1);
+    faces_in(d_phi, phi_i);
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     HIP_OK(hipMemsetAsync(mom_flags, 0, 4 * sizeof(int), stream));
-    viscosity_check_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, d_mu, mom_flags, err);
+    viscosity_check_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, mu_i, mom_flags, err);
     nonorth_scan_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, patch, npatch, patch_kind, d_vbt, fc,
                                                              fs, cc, mom_flags, err);
     wls_gradient_kernel<3><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
                                                                 patch_kind, d_vbt, d_vbv, 3, fc, fs, cc, vol,
-                                                                stencil_dimension(), 1.e8, d_U, mom_ugrad, nullptr,
+                                                                stencil_dimension(), 1.e8, u_i, mom_ugrad, nullptr,
                                                                 mom_flags, err);
     momentum_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind, d_vbt,
-                                                              fc, fs, cc, vol, d_mu, d_phi, mom_ugrad,
+                                                              fc, fs, cc, vol, mu_i, phi_i, mom_ugrad,
                                                               bounded ? 1 : 0, mom_flags, mom_aon, mom_ano, mom_nof,
                                                               err);
     momentum_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(
-        nc, nf, coff, cfaces_sorted ? cfaces_sorted : cfaces, owner, neigh, patch, npatch, patch_kind, d_vbt, d_vbv,
-        vol, d_phi, mom_aon, mom_ano, mom_nof, d_gradp, d_body, d_U, bounded ? 1 : 0, mom_flags,
-        alpha != 1.0 ? 1 : 0, alpha, d_diag, d_rhs, err);
-    momentum_values_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, plan.nzf_off, plan.nzf,
-                                                                owner, d_diag, mom_aon, mom_ano, d_vals);
+        nc, nf, coff, cfaces_sorted, owner, neigh, patch, npatch, patch_kind, d_vbt, d_vbv, vol, phi_i, mom_aon,
+        mom_ano, mom_nof, gp_i, body_i, u_i, bounded ? 1 : 0, mom_flags, alpha != 1.0 ? 1 : 0, alpha, diag_i, rhs_i,
+        err);
+    momentum_values_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, cell_new, plan.nzf_off,
+                                                                plan.nzf, owner, diag_i, mom_aon, mom_ano, d_vals);
+    cells_out(diag_i, d_diag, 1);
+    cells_out(rhs_i, d_rhs, 3);
     HIP_OK(hipStreamSynchronize(stream));
     check_err("assemble_momentum_component: invalid viscosity, flux or geometry, or non-positive diagonal");
     return plan;
@@ -1541,15 +1718,15 @@ void FvMesh::momentum_hbya_device(const double* d_vals, const double* d_rhs, con
                                   const double* d_gradp, const double* d_U, double* d_rau, double* d_hbya) {
     const PressurePlan& plan = pressure_plan(false, 0);
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
-    hbya_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, d_vals, d_rhs, d_diag, d_gradp, vol,
-                                                     d_U, d_rau, d_hbya, err);
+    hbya_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, d_vals, d_rhs, d_diag, d_gradp,
+                                                     vol_caller, d_U, d_rau, d_hbya, err);
     HIP_OK(hipStreamSynchronize(stream));
     check_err("HbyA: invalid momentum diagonal or non-finite HbyA");
 }
 
 void FvMesh::velocity_correct_device(const double* d_hbya, const double* d_rau, const double* d_gradp,
                                      double* d_U) {
-    velocity_correct_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, d_hbya, d_rau, d_gradp, vol, d_U);
+    velocity_correct_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, d_hbya, d_rau, d_gradp, vol_caller, d_U);
     HIP_OK(hipStreamSynchronize(stream));
 }
 
@@ -1566,10 +1743,18 @@ void FvMesh::rhie_chow_flux_device(const uint8_t* h_vbt, const double* h_vbv, co
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     positive_check_kernel<<<grid_for(3 * uint64_t(nc)), kBlock, 0, stream>>>(3 * uint64_t(nc), d_rau, 53, err);
-    gauss_gradient_device(d_pbt, d_pbv, d_p, work_grad);
+    double* h_i = scr(6, 3 * uint64_t(nc));
+    double* rau_i = scr(5, 3 * uint64_t(nc));
+    double* p_i = scr(9, nc);
+    double* phi_i = scr(4, nf);
+    cells_in(d_hbya, h_i, 3);
+    cells_in(d_rau, rau_i, 3);
+    cells_in(d_p, p_i, 1);
+    gauss_gradient_internal(d_pbt, d_pbv, p_i, work_grad);
     rhie_chow_flux_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind,
-                                                               d_vbt, d_vbv, d_pbt, d_pbv, rho, fc, fs, cc, vol,
-                                                               d_hbya, d_p, work_grad, d_rau, d_phi, err);
+                                                               d_vbt, d_vbv, d_pbt, d_pbv, rho, fc, fs, cc, vol, h_i,
+                                                               p_i, work_grad, rau_i, phi_i, err);
+    faces_out(phi_i, d_phi);
     HIP_OK(hipStreamSynchronize(stream));
     check_err("make_rhie_chow_mass_flux: inverse momentum diagonal must be finite and positive, or invalid face");
 }

@@ -33,7 +33,10 @@ struct FvMesh {
     uint32_t np = 0, nf = 0, nc = 0, npatch = 0;
     double *px = nullptr, *py = nullptr, *pz = nullptr;
     uint32_t *foff = nullptr, *fvtx = nullptr, *owner = nullptr, *coff = nullptr, *cfaces = nullptr;
-    uint32_t* cfaces_sorted = nullptr;  // null when cfaces is already ascending per cell
+    uint32_t* cfaces_sorted = nullptr;  // each cell's faces in ascending caller face id (internal ids)
+    // internal numbering: new_of_old / old_of_new for cells and faces
+    uint32_t *cell_new = nullptr, *cell_old = nullptr, *face_new = nullptr, *face_old = nullptr;
+    std::vector<uint32_t> h_cell_new, h_face_new;
     int32_t* neigh = nullptr;
     uint32_t* patch = nullptr;
     uint8_t* patch_kind = nullptr;
@@ -42,6 +45,7 @@ struct FvMesh {
     double *fsx = nullptr, *fsy = nullptr, *fsz = nullptr;
     double *ccx = nullptr, *ccy = nullptr, *ccz = nullptr;
     double* vol = nullptr;
+    double* vol_caller = nullptr;  // cell volumes in the caller's numbering
     int* err = nullptr;
     // work
     double *work_coeff = nullptr, *work_rc = nullptr, *work_bsrc = nullptr, *work_diag = nullptr,
@@ -61,6 +65,9 @@ struct FvMesh {
     std::vector<uint32_t> h_owner, h_coff, h_cf, h_patch;
     std::vector<int32_t> h_neigh;
     std::vector<uint8_t> h_kind;
+    // scratch for fields moved into the internal numbering (slot -> buffer)
+    std::vector<std::pair<double*, uint64_t>> scratch;
+    uint8_t* scratch_u8 = nullptr;
 
     FvMesh() = default;
     ~FvMesh();
@@ -100,6 +107,12 @@ struct FvMesh {
     void upload_bcs(const uint8_t* h_vbt, const double* h_vbv, const uint8_t* h_pbt, const double* h_pbv);
     int stencil_dimension() const;
     void ensure_work();
+    double* scr(size_t slot, uint64_t count);
+    void cells_in(const double* caller, double* internal, int comps);
+    void cells_out(const double* internal, double* caller, int comps);
+    void faces_in(const double* caller, double* internal);
+    void faces_out(const double* internal, double* caller);
+    void gauss_gradient_internal(const uint8_t* d_bct, const double* d_bcv, const double* p_i, double* grad_i);
     void check_err(const char* what);
     void release();
 };
