@@ -178,32 +178,87 @@ __global__ __launch_bounds__(kBlock) void cell_volume_kernel(uint32_t nc, const 
 
 // ---------------------------------------------------------- gradient ----
 
-// bc arrays are per patch: type 0 none/zero-gradient, 1 fixed value,
-// 2 fixed gradient; kind 1 = EMPTY patch (face skipped).
+// Slot stencils: geometry-only data of the cell-wise face loops, built once
+// per mesh in slot-major order (entry q * nc + c belongs to the q-th face of
+// cell c in its face-list order). A cell loop then reads them coalesced
+// instead of gathering the face and neighbour-cell arrays face by face.
+//   other: the cell across an internal face, -1 for a boundary face, -2 for
+//          an EMPTY face or an unused slot;
+//   face:  the face;
+//   w:     the distance weight dn / (dc + dn) of an internal face (the
+//          Green-Gauss face interpolation);
+//   s:     the face area vector oriented out of the cell.
+struct Slots {
+    int32_t* other;
+    uint32_t* face;
+    double* w;
+    Soa3 s;
+};
+
+__global__ __launch_bounds__(kBlock) void slot_build_kernel(uint32_t nc, uint32_t D, const uint32_t* __restrict__ coff,
+                                                           const uint32_t* __restrict__ cf,
+                                                           const uint32_t* __restrict__ owner,
+                                                           const int32_t* __restrict__ neigh,
+                                                           const uint32_t* __restrict__ patch, uint32_t npatch,
+                                                           const uint8_t* __restrict__ kind, Soa3 fc, Soa3 fs,
+                                                           Soa3 cc, Slots sl, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const uint32_t deg = coff[c + 1] - coff[c];
+    const D3 centre = cc.get(c);
+    for (uint32_t q = 0; q < D; ++q) {
+        const size_t idx = size_t(q) * nc + c;
+        if (q >= deg) {
+            sl.other[idx] = -2;
+            sl.face[idx] = 0;
+            sl.w[idx] = 0.0;
+            sl.s.set(idx, d3(0.0, 0.0, 0.0));
+            continue;
+        }
+        const uint32_t f = cf[coff[c] + q];
+        const bool own = owner[f] == c;
+        sl.face[idx] = f;
+        sl.s.set(idx, own ? fs.get(f) : mul3(fs.get(f), -1.0));
+        if (neigh[f] >= 0) {
+            const uint32_t other = own ? uint32_t(neigh[f]) : owner[f];
+            const double dc = mag3(sub3(fc.get(f), centre));
+            const double dn = mag3(sub3(cc.get(other), fc.get(f)));
+            if (!(dc + dn > 0.0) || !isfinite(dc + dn)) err[0] = 30;
+            sl.other[idx] = int32_t(other);
+            sl.w[idx] = dn / (dc + dn);
+        } else {
+            const uint32_t pp = patch[f];
+            sl.other[idx] = pp < npatch && kind[pp] == 1 ? -2 : -1;
+            sl.w[idx] = 0.0;
+        }
+    }
+}
+
+// gauss_gradient_with_boundary over the slot stencils. bc arrays are per
+// patch: type 0 none/zero-gradient, 1 fixed value, 2 fixed gradient.
 __global__ __launch_bounds__(kBlock) void gauss_gradient_kernel(
-    uint32_t nc, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cf,
-    const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh, const uint32_t* __restrict__ patch,
-    uint32_t npatch, const uint8_t* __restrict__ kind, const uint8_t* __restrict__ bct,
-    const double* __restrict__ bcv, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
+    uint32_t nc, const uint32_t* __restrict__ coff, Slots sl, const uint32_t* __restrict__ patch, uint32_t npatch,
+    const uint8_t* __restrict__ bct, const double* __restrict__ bcv, Soa3 fc, Soa3 cc, const double* __restrict__ vol,
     const double* __restrict__ p, double* __restrict__ grad) {
 #pragma clang fp contract(off)
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
+    const uint32_t deg = coff[c + 1] - coff[c];
+    const double pc = p[c];
     D3 sum = d3(0.0, 0.0, 0.0);
-    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
-        const uint32_t f = cf[k];
-        const bool own = owner[f] == c;
-        const D3 Sf = own ? fs.get(f) : mul3(fs.get(f), -1.0);
-        double vf = p[c];
-        if (neigh[f] >= 0) {
-            const uint32_t other = own ? uint32_t(neigh[f]) : owner[f];
-            const double dc = mag3(sub3(fc.get(f), cc.get(c)));
-            const double dn = mag3(sub3(cc.get(other), fc.get(f)));
-            const double w = dn / (dc + dn);
-            vf = w * p[c] + (1.0 - w) * p[other];
+    for (uint32_t q = 0; q < deg; ++q) {
+        const size_t idx = size_t(q) * nc + c;
+        const int32_t o = sl.other[idx];
+        if (o == -2) continue;
+        const D3 Sf = sl.s.get(idx);
+        double vf = pc;
+        if (o >= 0) {
+            const double w = sl.w[idx];
+            vf = w * pc + (1.0 - w) * p[o];
         } else {
+            const uint32_t f = sl.face[idx];
             const uint32_t pp = patch[f];
-            if (pp < npatch && kind[pp] == 1) continue;
             if (pp < npatch) {
                 if (bct[pp] == 1) {
                     vf = bcv[pp];
@@ -518,113 +573,53 @@ __device__ __forceinline__ bool boundary_normal_distance(D3 Sf, D3 Cf, D3 Cc, do
     return true;
 }
 
-// K scalar fields at once (the velocity components share the stencil). Field
-// k's boundary value at patch p is bcv[p * bstride + k]; bct: 0 zero-gradient,
-// 1 fixed value, 2 fixed gradient. Green-Gauss first (kept where the WLS fit
-// is not certified), then the one-sided-Jacobi WLS fit.
-template <int K>
-__global__ __launch_bounds__(kBlock) void wls_gradient_kernel(
-    uint32_t nc, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cf,
-    const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh, const uint32_t* __restrict__ patch,
-    uint32_t npatch, const uint8_t* __restrict__ kind, const uint8_t* __restrict__ bct,
-    const double* __restrict__ bcv, uint32_t bstride, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
-    int dim, double limit, const double* __restrict__ field, double* __restrict__ grad,
-    uint8_t* __restrict__ fallback, const int* __restrict__ gate, int* __restrict__ err) {
+// WLS fit cache. The weighted design matrix of compute_boundary_aware_wls_gradient
+// depends on the geometry and on which boundary faces are fixed-value (their
+// sample is the face centre, otherwise the centre's normal projection), not
+// on the field. wls_build_kernel runs the one-sided Jacobi factorisation once
+// per cell and keeps, slot-major, each sample's rotated design row (a0, a1,
+// a2), its weight 1/|d| (fac) and its normal distance (nd, boundary samples),
+// and per cell the singular values (sig, 3 x nc), the right singular vectors
+// (V, 9 x nc, row-major per cell), the condition estimate (cond) and meta:
+// bit 0 geometry finite, bit 1 rank == requested dimension, bits 2-7 the
+// descending singular-value order (2 bits each).
+struct WlsCache {
+    double *a0, *a1, *a2, *fac, *nd;
+    double *sig, *V, *cond;
+    uint32_t* meta;
+};
+
+__global__ __launch_bounds__(kBlock) void wls_build_kernel(uint32_t nc, const uint32_t* __restrict__ coff, Slots sl,
+                                                          const uint32_t* __restrict__ patch, uint32_t npatch,
+                                                          const uint8_t* __restrict__ bct, Soa3 fc, Soa3 cc, int dim,
+                                                          WlsCache wc, int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
-    if (gate && !(gate[0] && gate[1])) return;  // momentum path: only when a face is non-orthogonal
-    const size_t n1 = nc, n3 = 3 * size_t(nc);
-    const uint32_t k0 = coff[c], k1 = coff[c + 1];
+    const size_t n1 = nc;
+    const uint32_t deg = coff[c + 1] - coff[c];
     const D3 centre = cc.get(c);
-    const double V = vol[c];
-    double value[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) value[k] = field[k * n1 + c];
-    // Green-Gauss with boundary values (compute_scalar_gradient_with_boundary).
-    {
-        D3 sum[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) sum[k] = d3(0.0, 0.0, 0.0);
-        for (uint32_t q = k0; q < k1; ++q) {
-            const uint32_t f = cf[q];
-            const bool own = owner[f] == c;
-            const D3 Sf = own ? fs.get(f) : mul3(fs.get(f), -1.0);
-            double fv[K];
-            if (neigh[f] >= 0) {
-                const uint32_t other = own ? uint32_t(neigh[f]) : owner[f];
-                const double dc = mag3(sub3(fc.get(f), centre));
-                const double dn = mag3(sub3(cc.get(other), fc.get(f)));
-                if (!(dc + dn > 0.0) || !isfinite(dc + dn)) {
-                    err[0] = 30;
-                    return;
-                }
-                const double w = dn / (dc + dn);
-#pragma unroll
-                for (int k = 0; k < K; ++k) fv[k] = w * value[k] + (1.0 - w) * field[k * n1 + other];
-            } else {
-                const uint32_t pp = patch[f];
-                if (pp < npatch && kind[pp] == 1) continue;
-                const uint8_t t = pp < npatch ? bct[pp] : 0;
-                double dn = 0.0;
-                if (t == 2 && !boundary_normal_distance(fs.get(f), fc.get(f), centre, V, dn)) {
-                    err[0] = 31;
-                    return;
-                }
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    fv[k] = value[k];
-                    if (t == 1) fv[k] = bcv[size_t(pp) * bstride + k];
-                    else if (t == 2) fv[k] = value[k] + bcv[size_t(pp) * bstride + k] * dn;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (!isfinite(fv[k])) {
-                    err[0] = 32;
-                    return;
-                }
-                sum[k] = add3(sum[k], mul3(Sf, fv[k]));
-            }
-        }
-        if (!(V > 0.0) || !isfinite(V)) {
-            err[0] = 33;
-            return;
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            grad[k * n3 + c] = sum[k].x / V;
-            grad[k * n3 + n1 + c] = sum[k].y / V;
-            grad[k * n3 + 2 * n1 + c] = sum[k].z / V;
-        }
-    }
-    // Weighted design matrix rows (1/|d| weights) and right-hand sides.
-    if (k1 - k0 > uint32_t(kMaxCellFaces)) {
+    if (deg > uint32_t(kMaxCellFaces)) {
         err[0] = 34;
         return;
     }
     double A[kMaxCellFaces][3];
-    double B[kMaxCellFaces][K];
-    bool ok[K];
+    int slot_of[kMaxCellFaces];
     bool geo_ok = isfinite(centre.x) && isfinite(centre.y) && isfinite(centre.z);
-#pragma unroll
-    for (int k = 0; k < K; ++k) ok[k] = isfinite(value[k]);
     int ns = 0;
-    for (uint32_t q = k0; q < k1; ++q) {
-        const uint32_t f = cf[q];
-        const bool own = owner[f] == c;
+    for (uint32_t q = 0; q < deg; ++q) {
+        const size_t idx = size_t(q) * nc + c;
+        const int32_t o = sl.other[idx];
+        if (o == -2) continue;
         D3 pt;
-        double sv[K];
-        if (neigh[f] >= 0) {
-            const uint32_t other = own ? uint32_t(neigh[f]) : owner[f];
-            pt = cc.get(other);
-#pragma unroll
-            for (int k = 0; k < K; ++k) sv[k] = field[k * n1 + other];
+        if (o >= 0) {
+            pt = cc.get(uint32_t(o));
+            wc.nd[idx] = 0.0;
         } else {
+            const uint32_t f = sl.face[idx];
             const uint32_t pp = patch[f];
-            if (pp < npatch && kind[pp] == 1) continue;
             const uint8_t t = pp < npatch ? bct[pp] : 0;
-            const D3 sf = own ? fs.get(f) : mul3(fs.get(f), -1.0);
+            const D3 sf = sl.s.get(idx);
             const double area = mag3(sf);
             if (!(area > 0.0) || !isfinite(area)) {
                 err[0] = 35;
@@ -636,40 +631,30 @@ __global__ __launch_bounds__(kBlock) void wls_gradient_kernel(
                 err[0] = 36;
                 return;
             }
-            if (t == 1) {
-                pt = fc.get(f);
-#pragma unroll
-                for (int k = 0; k < K; ++k) sv[k] = bcv[size_t(pp) * bstride + k];
-            } else {
-                pt = add3(centre, mul3(normal, nd));
-#pragma unroll
-                for (int k = 0; k < K; ++k) sv[k] = value[k] + nd * (t == 2 ? bcv[size_t(pp) * bstride + k] : 0.0);
-            }
+            wc.nd[idx] = nd;
+            pt = t == 1 ? fc.get(f) : add3(centre, mul3(normal, nd));
         }
         const D3 d = sub3(pt, centre);
         const double r2 = mag2_3(d);
         if (!(r2 > 0.0) || !isfinite(r2)) geo_ok = false;
         const double factor = 1.0 / sqrt(r2);
         if (!isfinite(factor)) geo_ok = false;
+        wc.fac[idx] = factor;
 #pragma unroll
         for (int i = 0; i < kMaxCellFaces; ++i)
             if (i == ns) {
                 A[i][0] = factor * d.x;
                 A[i][1] = factor * d.y;
                 A[i][2] = factor * d.z;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const double dv = sv[k] - value[k];
-                    if (!isfinite(dv)) ok[k] = false;
-                    B[i][k] = factor * dv;
-                }
+                slot_of[i] = int(q);
             }
         ++ns;
     }
-    bool certified = false;
     double Vm[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
     double sigma[3] = {0.0, 0.0, 0.0};
     int order[3] = {0, 1, 2};
+    bool rank_ok = false;
+    double cond = INFINITY;
     if (geo_ok) {
         constexpr double eps = 2.220446049250313e-16;
         for (int sweep = 0; sweep < 40; ++sweep) {
@@ -744,34 +729,170 @@ __global__ __launch_bounds__(kBlock) void wls_gradient_kernel(
         if (dim == 2 && rank == 3 && sigma[order[2]] <= 1e-10 * sigma[order[0]]) rank = 2;
         const bool full_rank = rank == dim;
         const double min_pivot = full_rank ? sigma[order[dim - 1]] : 0.0;
-        const double cond = full_rank && min_pivot > 0.0 ? scale / min_pivot : INFINITY;
-        certified = rank == dim && full_rank && rank > 0 && isfinite(cond) && cond <= limit;
+        cond = full_rank && min_pivot > 0.0 ? scale / min_pivot : INFINITY;
+        rank_ok = full_rank && rank > 0;
     }
-    bool any_fallback = false;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        bool use = certified && ok[k];
-        double g[3] = {0.0, 0.0, 0.0};
-        if (use) {
-            for (int oi = 0; oi < dim; ++oi) {
-                const int j = order[oi];
-                double proj = 0.0;
-#pragma unroll
-                for (int i = 0; i < kMaxCellFaces; ++i)
-                    if (i < ns) proj += (j == 0 ? A[i][0] : (j == 1 ? A[i][1] : A[i][2])) * B[i][k];
-                const double sj = sigma[j];
-                const double coeff = proj / (sj * sj);
-#pragma unroll
-                for (int r = 0; r < 3; ++r) g[r] += (j == 0 ? Vm[r][0] : (j == 1 ? Vm[r][1] : Vm[r][2])) * coeff;
-            }
-            use = isfinite(g[0]) && isfinite(g[1]) && isfinite(g[2]);
+    for (int i = 0; i < kMaxCellFaces; ++i)
+        if (i < ns) {
+            const size_t idx = size_t(slot_of[i]) * nc + c;
+            wc.a0[idx] = A[i][0];
+            wc.a1[idx] = A[i][1];
+            wc.a2[idx] = A[i][2];
         }
-        if (use) {
-            grad[k * n3 + c] = g[0];
-            grad[k * n3 + n1 + c] = g[1];
-            grad[k * n3 + 2 * n1 + c] = g[2];
-        } else {
-            any_fallback = true;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) wc.sig[j * n1 + c] = sigma[j];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) wc.V[(3 * r + j) * n1 + c] = Vm[r][j];
+    wc.cond[c] = cond;
+    wc.meta[c] = (geo_ok ? 1u : 0u) | (rank_ok ? 2u : 0u) | (uint32_t(order[0]) << 2) |
+                 (uint32_t(order[1]) << 4) | (uint32_t(order[2]) << 6);
+}
+
+// K scalar fields at once (the velocity components share the stencil): the
+// boundary-aware Green-Gauss gradient (compute_scalar_gradient_with_boundary),
+// replaced by the cached WLS fit where it is certified. Field k's boundary
+// value at patch p is bcv[p * bstride + k]; bct: 0 zero-gradient, 1 fixed
+// value, 2 fixed gradient. gate (momentum path): skip the whole pass unless
+// diffusion is active and a face is non-orthogonal.
+template <int K>
+__global__ __launch_bounds__(kBlock) void wls_gradient_kernel(
+    uint32_t nc, const uint32_t* __restrict__ coff, Slots sl, const uint32_t* __restrict__ patch, uint32_t npatch,
+    const uint8_t* __restrict__ bct, const double* __restrict__ bcv, uint32_t bstride, Soa3 fc, Soa3 fs, Soa3 cc,
+    const double* __restrict__ vol, int dim, double limit, WlsCache wc, const double* __restrict__ field,
+    double* __restrict__ grad, uint8_t* __restrict__ fallback, const int* __restrict__ gate, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    if (gate && !(gate[0] && gate[1])) return;
+    const size_t n1 = nc, n3 = 3 * size_t(nc);
+    const uint32_t deg = coff[c + 1] - coff[c];
+    const D3 centre = cc.get(c);
+    const double V = vol[c];
+    double value[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) value[k] = field[k * n1 + c];
+    // Green-Gauss with boundary values (compute_scalar_gradient_with_boundary).
+    {
+        D3 sum[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) sum[k] = d3(0.0, 0.0, 0.0);
+        for (uint32_t q = 0; q < deg; ++q) {
+            const size_t idx = size_t(q) * nc + c;
+            const int32_t o = sl.other[idx];
+            if (o == -2) continue;
+            const D3 Sf = sl.s.get(idx);
+            double fv[K];
+            if (o >= 0) {
+                const double w = sl.w[idx];
+#pragma unroll
+                for (int k = 0; k < K; ++k) fv[k] = w * value[k] + (1.0 - w) * field[k * n1 + uint32_t(o)];
+            } else {
+                const uint32_t f = sl.face[idx];
+                const uint32_t pp = patch[f];
+                const uint8_t t = pp < npatch ? bct[pp] : 0;
+                double dn = 0.0;
+                if (t == 2 && !boundary_normal_distance(fs.get(f), fc.get(f), centre, V, dn)) {
+                    err[0] = 31;
+                    return;
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    fv[k] = value[k];
+                    if (t == 1) fv[k] = bcv[size_t(pp) * bstride + k];
+                    else if (t == 2) fv[k] = value[k] + bcv[size_t(pp) * bstride + k] * dn;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (!isfinite(fv[k])) {
+                    err[0] = 32;
+                    return;
+                }
+                sum[k] = add3(sum[k], mul3(Sf, fv[k]));
+            }
+        }
+        if (!(V > 0.0) || !isfinite(V)) {
+            err[0] = 33;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            grad[k * n3 + c] = sum[k].x / V;
+            grad[k * n3 + n1 + c] = sum[k].y / V;
+            grad[k * n3 + 2 * n1 + c] = sum[k].z / V;
+        }
+    }
+    // The cached WLS fit: rhs_i = fac_i (value_i - value); proj_j = sum_i
+    // a_ij rhs_i (sample order); g += V e_j proj_j / sigma_j^2 (descending).
+    const uint32_t meta = wc.meta[c];
+    const double cond = wc.cond[c];
+    const bool certified = (meta & 1u) && (meta & 2u) && isfinite(cond) && cond <= limit;
+    bool any_fallback = !certified;
+    if (certified) {
+        const int o0 = int((meta >> 2) & 3u), o1 = int((meta >> 4) & 3u), o2 = int((meta >> 6) & 3u);
+        bool ok[K];
+        double proj[K][3];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ok[k] = isfinite(value[k]);
+            proj[k][0] = proj[k][1] = proj[k][2] = 0.0;
+        }
+        for (uint32_t q = 0; q < deg; ++q) {
+            const size_t idx = size_t(q) * nc + c;
+            const int32_t o = sl.other[idx];
+            if (o == -2) continue;
+            const double fac = wc.fac[idx];
+            const double a[3] = {wc.a0[idx], wc.a1[idx], wc.a2[idx]};
+            const double aj0 = o0 == 0 ? a[0] : (o0 == 1 ? a[1] : a[2]);
+            const double aj1 = o1 == 0 ? a[0] : (o1 == 1 ? a[1] : a[2]);
+            const double aj2 = o2 == 0 ? a[0] : (o2 == 1 ? a[1] : a[2]);
+            uint8_t t = 0;
+            uint32_t pp = 0;
+            double nd = 0.0;
+            if (o < 0) {
+                pp = patch[sl.face[idx]];
+                t = pp < npatch ? bct[pp] : 0;
+                nd = wc.nd[idx];
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                double sv;
+                if (o >= 0) sv = field[k * n1 + uint32_t(o)];
+                else if (t == 1) sv = bcv[size_t(pp) * bstride + k];
+                else sv = value[k] + nd * (t == 2 ? bcv[size_t(pp) * bstride + k] : 0.0);
+                const double dv = sv - value[k];
+                if (!isfinite(dv)) ok[k] = false;
+                const double rhs = fac * dv;
+                proj[k][0] += aj0 * rhs;
+                proj[k][1] += aj1 * rhs;
+                proj[k][2] += aj2 * rhs;
+            }
+        }
+        const int ord[3] = {o0, o1, o2};
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            bool use = ok[k];
+            double g[3] = {0.0, 0.0, 0.0};
+            if (use) {
+                for (int oi = 0; oi < dim; ++oi) {
+                    const int j = ord[oi];
+                    const double sj = wc.sig[j * n1 + c];
+                    const double coeff = proj[k][oi] / (sj * sj);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) g[r] += wc.V[(3 * r + j) * n1 + c] * coeff;
+                }
+                use = isfinite(g[0]) && isfinite(g[1]) && isfinite(g[2]);
+            }
+            if (use) {
+                grad[k * n3 + c] = g[0];
+                grad[k * n3 + n1 + c] = g[1];
+                grad[k * n3 + 2 * n1 + c] = g[2];
+            } else {
+                any_fallback = true;
+            }
         }
     }
     if (fallback) fallback[c] = any_fallback ? 1 : 0;
@@ -1172,12 +1293,22 @@ __global__ __launch_bounds__(kBlock) void rhie_chow_flux_kernel(
 // (cfaces) and visits faces in ascending caller face id where the reference
 // accumulates in face order (cfaces_sorted), so every value is unchanged.
 
+namespace {
+Slots slots_of(const FvMesh& m) {
+    return Slots{m.slot_other, m.slot_face, m.slot_w, Soa3{m.slot_sx, m.slot_sy, m.slot_sz}};
+}
+WlsCache wls_of(const FvMesh& m) {
+    return WlsCache{m.wls_a0, m.wls_a1, m.wls_a2, m.wls_fac, m.wls_nd, m.wls_sig, m.wls_V, m.wls_cond, m.wls_meta};
+}
+}  // namespace
+
 FvMesh::~FvMesh() { release(); }
 
 void FvMesh::release() {
     for (double** p : {&px, &py, &pz, &fcx, &fcy, &fcz, &fsx, &fsy, &fsz, &ccx, &ccy, &ccz, &vol, &vol_caller,
                        &work_coeff, &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi,
-                       &mom_aon, &mom_ano, &mom_nof, &mom_ugrad, &bc_f64})
+                       &mom_aon, &mom_ano, &mom_nof, &mom_ugrad, &bc_f64, &slot_w, &slot_sx, &slot_sy, &slot_sz,
+                       &wls_a0, &wls_a1, &wls_a2, &wls_fac, &wls_nd, &wls_sig, &wls_V, &wls_cond})
         dfree(*p);
     for (uint32_t** p : {&foff, &fvtx, &owner, &coff, &cfaces, &cfaces_sorted, &patch, &cell_new, &cell_old,
                          &face_new, &face_old})
@@ -1185,6 +1316,9 @@ void FvMesh::release() {
     for (auto& b : scratch) dfree(b.first);
     scratch.clear();
     dfree(scratch_u8);
+    dfree(slot_other);
+    dfree(slot_face);
+    dfree(wls_meta);
     dfree(neigh);
     dfree(patch_kind);
     dfree(err);
@@ -1395,11 +1529,59 @@ void FvMesh::ensure_work() {
     fixed_face = dalloc<uint8_t>(nf);
 }
 
-void FvMesh::gauss_gradient_internal(const uint8_t* d_bct, const double* d_bcv, const double* p_i, double* grad_i) {
+// Slot stencils (geometry only): built on first use.
+void FvMesh::ensure_slots() {
+    if (slot_other) return;
+    uint32_t D = 0;
+    for (uint32_t c = 0; c < nc; ++c) D = std::max(D, h_coff[c + 1] - h_coff[c]);
+    slot_D = D;
+    const uint64_t m = uint64_t(D) * nc;
+    slot_other = dalloc<int32_t>(m);
+    slot_face = dalloc<uint32_t>(m);
+    slot_w = dalloc<double>(m);
+    slot_sx = dalloc<double>(m);
+    slot_sy = dalloc<double>(m);
+    slot_sz = dalloc<double>(m);
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
-    gauss_gradient_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
-                                                               patch_kind, d_bct, d_bcv, fc, fs, cc, vol, p_i,
-                                                               grad_i);
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    slot_build_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, D, coff, cfaces, owner, neigh, patch, npatch,
+                                                           patch_kind, fc, fs, cc, slots_of(*this), err);
+    HIP_OK(hipStreamSynchronize(stream));
+    check_err("cell stencils: degenerate face distances");
+}
+
+// WLS fit cache for one set of boundary types (fixed value or not, per patch).
+void FvMesh::ensure_wls_cache(const uint8_t* h_bt, const uint8_t* d_bt) {
+    ensure_slots();
+    std::vector<uint8_t> key(npatch);
+    for (uint32_t p = 0; p < npatch; ++p) key[p] = h_bt[p] == 1 ? 1 : 0;
+    if (wls_meta && key == wls_key) return;
+    if (slot_D > uint32_t(kMaxCellFaces))
+        throw std::invalid_argument("boundary WLS: cells with more than " + std::to_string(kMaxCellFaces) +
+                                    " faces are not supported");
+    const uint64_t m = uint64_t(slot_D) * nc;
+    if (!wls_meta) {
+        for (double** p : {&wls_a0, &wls_a1, &wls_a2, &wls_fac, &wls_nd}) *p = dalloc<double>(m);
+        wls_sig = dalloc<double>(3 * uint64_t(nc));
+        wls_V = dalloc<double>(9 * uint64_t(nc));
+        wls_cond = dalloc<double>(nc);
+        wls_meta = dalloc<uint32_t>(nc);
+    }
+    wls_key.clear();
+    const Soa3 fc{fcx, fcy, fcz}, cc{ccx, ccy, ccz};
+    HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
+    wls_build_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, slots_of(*this), patch, npatch, d_bt, fc, cc,
+                                                          stencil_dimension(), wls_of(*this), err);
+    HIP_OK(hipStreamSynchronize(stream));
+    check_err("boundary WLS: degenerate boundary face or normal distance");
+    wls_key = key;
+}
+
+void FvMesh::gauss_gradient_internal(const uint8_t* d_bct, const double* d_bcv, const double* p_i, double* grad_i) {
+    ensure_slots();
+    const Soa3 fc{fcx, fcy, fcz}, cc{ccx, ccy, ccz};
+    gauss_gradient_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, slots_of(*this), patch, npatch, d_bct, d_bcv, fc, cc,
+                                                               vol, p_i, grad_i);
 }
 
 void FvMesh::gauss_gradient_device(const uint8_t* d_bct, const double* d_bcv, const double* d_p, double* d_grad) {
@@ -1650,11 +1832,17 @@ void FvMesh::wls_gradient_device(const uint8_t* h_bt, const double* h_bv, double
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     double* f_i = scr(0, nc);
     double* g_i = scr(8, 3 * uint64_t(nc));
+    try {
+        ensure_wls_cache(h_bt, d_bt);
+    } catch (...) {
+        dfree(d_bt);
+        dfree(d_bv);
+        throw;
+    }
     cells_in(d_field, f_i, 1);
-    wls_gradient_kernel<1><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
-                                                                patch_kind, d_bt, d_bv, 1, fc, fs, cc, vol,
-                                                                stencil_dimension(), limit, f_i, g_i, d_fallback,
-                                                                nullptr, err);
+    wls_gradient_kernel<1><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, slots_of(*this), patch, npatch, d_bt, d_bv, 1, fc,
+                                                                fs, cc, vol, stencil_dimension(), limit, wls_of(*this), f_i,
+                                                                g_i, d_fallback, nullptr, err);
     cells_out(g_i, d_grad, 3);
     HIP_OK(hipStreamSynchronize(stream));
     dfree(d_bt);
@@ -1675,6 +1863,7 @@ const PressurePlan& FvMesh::momentum_assemble_device(const uint8_t* h_vbt, const
     upload_bcs(h_vbt, h_vbv, nullptr, nullptr);
     const uint8_t* d_vbt = bc_u8;
     const double* d_vbv = bc_f64;
+    ensure_wls_cache(h_bc_u8.data(), d_vbt);
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     double* u_i = scr(0, 3 * uint64_t(nc));
     double* gp_i = scr(1, 3 * uint64_t(nc));
@@ -1693,10 +1882,9 @@ const PressurePlan& FvMesh::momentum_assemble_device(const uint8_t* h_vbt, const
     viscosity_check_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, mu_i, mom_flags, err);
     nonorth_scan_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, patch, npatch, patch_kind, d_vbt, fc,
                                                              fs, cc, mom_flags, err);
-    wls_gradient_kernel<3><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, owner, neigh, patch, npatch,
-                                                                patch_kind, d_vbt, d_vbv, 3, fc, fs, cc, vol,
-                                                                stencil_dimension(), 1.e8, u_i, mom_ugrad, nullptr,
-                                                                mom_flags, err);
+    wls_gradient_kernel<3><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, slots_of(*this), patch, npatch, d_vbt, d_vbv, 3, fc,
+                                                                fs, cc, vol, stencil_dimension(), 1.e8, wls_of(*this), u_i,
+                                                                mom_ugrad, nullptr, mom_flags, err);
     momentum_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind, d_vbt,
                                                               fc, fs, cc, vol, mu_i, phi_i, mom_ugrad,
                                                               bounded ? 1 : 0, mom_flags, mom_aon, mom_ano, mom_nof,

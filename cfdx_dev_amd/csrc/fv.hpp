@@ -65,6 +65,15 @@ struct FvMesh {
     std::vector<uint32_t> h_owner, h_coff, h_cf, h_patch;
     std::vector<int32_t> h_neigh;
     std::vector<uint8_t> h_kind;
+    // slot stencils (slot-major, slot_D slots per cell) and the WLS fit cache
+    uint32_t slot_D = 0;
+    int32_t* slot_other = nullptr;
+    uint32_t* slot_face = nullptr;
+    double *slot_w = nullptr, *slot_sx = nullptr, *slot_sy = nullptr, *slot_sz = nullptr;
+    double *wls_a0 = nullptr, *wls_a1 = nullptr, *wls_a2 = nullptr, *wls_fac = nullptr, *wls_nd = nullptr;
+    double *wls_sig = nullptr, *wls_V = nullptr, *wls_cond = nullptr;
+    uint32_t* wls_meta = nullptr;
+    std::vector<uint8_t> wls_key;
     // scratch for fields moved into the internal numbering (slot -> buffer)
     std::vector<std::pair<double*, uint64_t>> scratch;
     uint8_t* scratch_u8 = nullptr;
@@ -112,6 +121,8 @@ struct FvMesh {
     void cells_out(const double* internal, double* caller, int comps);
     void faces_in(const double* caller, double* internal);
     void faces_out(const double* internal, double* caller);
+    void ensure_slots();
+    void ensure_wls_cache(const uint8_t* h_bt, const uint8_t* d_bt);
     void gauss_gradient_internal(const uint8_t* d_bct, const double* d_bcv, const double* p_i, double* grad_i);
     void check_err(const char* what);
     void release();
