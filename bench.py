@@ -127,15 +127,13 @@ def momentum_bytes(m, nnz):
     return {"assembly": assembly, "hbya": hbya, "rhie_chow": rhie_chow, "correct": correct}
 
 
-def momentum_measure(s, hip, lib, reps, log):
-    """The momentum side of the SIMPLE corrector on the benchmark mesh, device
+def momentum_measure(fv, m, hip, lib, reps, log):
+    """The momentum side of the SIMPLE corrector on mesh m (handle fv), device
     pointers throughout (lid-driven boundary data, laminar viscosity, a
     random velocity and pressure): per-call seconds of each entry point and
     the achieved rate against momentum_bytes."""
     import ctypes as C
 
-    fv = s.extra["fv"]
-    m = s.mesh
     nc, nf = m.n_cells, m.n_faces
     rng = np.random.default_rng(11)
     vt = np.ones(m.n_patches, np.uint8)
@@ -390,7 +388,16 @@ def main():
 
     momentum = None
     if not args.no_momentum:
-        momentum = momentum_measure(s, hip, lib, args.momentum_repeat, log)
+        momentum = {"benchmark_numbering": momentum_measure(s.extra["fv"], s.mesh, hip, lib, args.momentum_repeat,
+                                                            log)}
+        # The same mesh numbered in lattice order (no cell permutation): what
+        # the kernels reach when the caller's numbering has locality.
+        layers = max(1, round(30 * args.n / 256))
+        ml = cx.generate_mesh("box", args.n, args.n, args.n, jitter=0.10, permute=False, seed=5, wall_layers=layers,
+                              first_aspect=1000.0)
+        fl = cx.FvMesh(ml)
+        momentum["lattice_numbering"] = momentum_measure(fl, ml, hip, lib, args.momentum_repeat, log)
+        del fl, ml
 
     cpu = None
     if not args.no_cpu:
@@ -462,7 +469,8 @@ def main():
         "level_kernels": level_rates,
         "momentum_side": None if momentum is None else {
             "what": "three relaxed momentum equations (UPWIND, WLS non-orthogonal correction, alpha_u 0.7), "
-                    "rAU/HbyA, Rhie-Chow flux, velocity correction; device pointers, per call",
+                    "rAU/HbyA, Rhie-Chow flux, velocity correction; device pointers, per call; on the benchmark "
+                    "mesh (randomly permuted cells) and on the same mesh in lattice order",
             **momentum},
         "roofline": {
             "bound": "hbm",

@@ -339,7 +339,9 @@ __global__ __launch_bounds__(kBlock) void pressure_face_kernel(
     const uint32_t* __restrict__ patch, uint32_t npatch, const uint8_t* __restrict__ pbt,
     const double* __restrict__ pbv, double rho, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
     const double* __restrict__ rAU, const double* __restrict__ p, const double* __restrict__ grad,
-    double* __restrict__ coeff, double* __restrict__ rcflux, double* __restrict__ bsrc, int* __restrict__ err) {
+    double* __restrict__ coeff, double* __restrict__ rcflux, double* __restrict__ bsrc,
+    const double* __restrict__ phi, const uint8_t* __restrict__ slot_o, const uint8_t* __restrict__ slot_n,
+    double* __restrict__ s_phi, double* __restrict__ s_rc, double* __restrict__ s_coeff, int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= nf) return;
@@ -348,6 +350,10 @@ __global__ __launch_bounds__(kBlock) void pressure_face_kernel(
     coeff[f] = 0.0;
     rcflux[f] = 0.0;
     bsrc[f] = 0.0;
+    // Continuity terms of this face in its cells' slots (EMPTY faces too: the
+    // reference's continuity loop visits every face of the cell).
+    const size_t io = size_t(slot_o[f]) * nc + o;
+    s_phi[io] = phi[f];
     if (neigh[f] >= 0) {
         const uint32_t n = uint32_t(neigh[f]);
         const double wn = 0.5, wo = 1.0 - wn;
@@ -365,6 +371,12 @@ __global__ __launch_bounds__(kBlock) void pressure_face_kernel(
         }
         coeff[f] = rho * ic;
         rcflux[f] = rho * (ic * (p[n] - p[o]) + nof);
+        const size_t in = size_t(slot_n[f]) * nc + n;
+        s_phi[in] = -phi[f];
+        s_rc[io] = -rcflux[f];
+        s_rc[in] = rcflux[f];
+        s_coeff[io] = coeff[f];
+        s_coeff[in] = coeff[f];
         return;
     }
     const uint32_t pp = patch[f];
@@ -382,35 +394,37 @@ __global__ __launch_bounds__(kBlock) void pressure_face_kernel(
     bsrc[f] = rho * (bc * (pbv[pp] - p[o]) + def);
 }
 
-// Per cell: continuity in cell-face order (±phiHbyA, ∓RC flux), diagonal =
-// internal coefficients then fixed-pressure boundary coefficients, both in
-// ascending face order; rhs = -continuity + boundary sources.
+// Per cell: continuity in cell-face list order (±phiHbyA, ∓RC flux),
+// diagonal = internal coefficients then fixed-pressure boundary
+// coefficients, both in ascending caller face id; rhs = -continuity +
+// boundary sources. Per-face terms come from the cell's slots.
 __global__ __launch_bounds__(kBlock) void pressure_cell_kernel(
-    uint32_t nc, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cf,
-    const uint32_t* __restrict__ cfs, const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh,
-    const double* __restrict__ phi, const double* __restrict__ coeff, const double* __restrict__ rcflux,
-    const double* __restrict__ bsrc, const uint8_t* __restrict__ fixed_face, double* __restrict__ diag,
-    double* __restrict__ rhs) {
+    uint32_t nc, const uint32_t* __restrict__ coff, const uint8_t* __restrict__ slot_sorted, Slots sl,
+    const double* __restrict__ s_phi, const double* __restrict__ s_rc, const double* __restrict__ s_coeff,
+    const double* __restrict__ coeff, const double* __restrict__ bsrc, const uint8_t* __restrict__ fixed_face,
+    double* __restrict__ diag, double* __restrict__ rhs) {
 #pragma clang fp contract(off)
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
+    const uint32_t deg = coff[c + 1] - coff[c];
     double cont = 0.0;
-    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
-        const uint32_t f = cf[k];
-        const bool own = owner[f] == c;
-        cont += own ? phi[f] : -phi[f];
-        if (neigh[f] >= 0) cont += own ? -rcflux[f] : rcflux[f];
+    for (uint32_t q = 0; q < deg; ++q) {
+        const size_t idx = size_t(q) * nc + c;
+        cont += s_phi[idx];
+        if (sl.other[idx] >= 0) cont += s_rc[idx];
     }
     double d = 0.0;
-    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
-        const uint32_t f = cfs[k];
-        if (neigh[f] >= 0) d += coeff[f];
+    for (uint32_t k = 0; k < deg; ++k) {
+        const size_t idx = size_t(slot_sorted[size_t(k) * nc + c]) * nc + c;
+        if (sl.other[idx] >= 0) d += s_coeff[idx];
     }
     double dd = 0.0 + d;
     double b = -cont;
-    for (uint32_t k = coff[c]; k < coff[c + 1]; ++k) {
-        const uint32_t f = cfs[k];
-        if (neigh[f] < 0 && fixed_face[f]) {
+    for (uint32_t k = 0; k < deg; ++k) {
+        const size_t idx = size_t(slot_sorted[size_t(k) * nc + c]) * nc + c;
+        if (sl.other[idx] >= 0) continue;
+        const uint32_t f = sl.face[idx];
+        if (fixed_face[f]) {
             dd += coeff[f];
             b += bsrc[f];
         }
@@ -942,6 +956,13 @@ __global__ __launch_bounds__(kBlock) void nonorth_scan_kernel(uint32_t nf, const
     if (mag3(tangential) > 1e-12 * area) flags[1] = 1;
 }
 
+// Per-slot momentum terms (slot-major): the face's diagonal coefficient for
+// this cell, its signed flux, its signed tangential fluxes of the three
+// components.
+struct MomSlots {
+    double *diag, *div, *d0, *d1, *d2;
+};
+
 // Per face: the upwind/diffusion coefficients the face adds to the owner's
 // diagonal (aon) and to the neighbour's diagonal, or for a fixed-value
 // boundary face the coefficient of the boundary value (ano); with needs, the
@@ -952,8 +973,8 @@ __global__ __launch_bounds__(kBlock) void momentum_face_kernel(
     const uint32_t* __restrict__ patch, uint32_t npatch, const uint8_t* __restrict__ kind,
     const uint8_t* __restrict__ vbt, Soa3 fc, Soa3 fs, Soa3 cc, const double* __restrict__ vol,
     const double* __restrict__ mu, const double* __restrict__ phi, const double* __restrict__ ugrad, int bounded,
-    const int* __restrict__ flags, double* __restrict__ aon, double* __restrict__ ano, double* __restrict__ nof,
-    int* __restrict__ err) {
+    const int* __restrict__ flags, const uint8_t* __restrict__ slot_o, const uint8_t* __restrict__ slot_n,
+    double* __restrict__ aon, double* __restrict__ ano, MomSlots ms, int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= nf) return;
@@ -1040,62 +1061,72 @@ __global__ __launch_bounds__(kBlock) void momentum_face_kernel(
     }
     aon[f] = t_on;
     ano[f] = t_no;
-    if (needs) {
-        nof[f] = flux[0];
-        nof[size_t(nf) + f] = flux[1];
-        nof[2 * size_t(nf) + f] = flux[2];
+    // This face's terms in the owner's slot and (internal faces) the
+    // neighbour's slot, signed for that side: x - t is x + (-t) exactly.
+    const size_t io = size_t(slot_o[f]) * nc + o;
+    ms.diag[io] = t_on;
+    ms.div[io] = F;
+    ms.d0[io] = flux[0];
+    ms.d1[io] = flux[1];
+    ms.d2[io] = flux[2];
+    if (neigh[f] >= 0) {
+        const size_t in = size_t(slot_n[f]) * nc + uint32_t(neigh[f]);
+        ms.diag[in] = t_no;
+        ms.div[in] = -F;
+        ms.d0[in] = -flux[0];
+        ms.d1[in] = -flux[1];
+        ms.d2[in] = -flux[2];
     }
 }
 
-// Per cell, faces in ascending id: diagonal, flux divergence, deferred
-// non-orthogonal terms and boundary sources exactly as the reference's face
-// loop accumulates them; then the bounded-convection correction, sources and
-// the equation relaxation (alpha < 1). diag: the relaxed diagonal (shared by
-// the three components); rhs: 3 x nc.
+// Per cell, faces in ascending caller face id (the reference's face loop
+// order): diagonal, flux divergence, deferred non-orthogonal terms and
+// boundary sources, read from the cell's slots; then the bounded-convection
+// correction, sources and the equation relaxation (alpha < 1). diag: the
+// relaxed diagonal (shared by the three components); rhs: 3 x nc.
 __global__ __launch_bounds__(kBlock) void momentum_cell_kernel(
-    uint32_t nc, uint32_t nf, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cfs,
-    const uint32_t* __restrict__ owner, const int32_t* __restrict__ neigh, const uint32_t* __restrict__ patch,
-    uint32_t npatch, const uint8_t* __restrict__ kind, const uint8_t* __restrict__ vbt,
+    uint32_t nc, const uint32_t* __restrict__ coff, const uint8_t* __restrict__ slot_sorted, Slots sl,
+    MomSlots ms, const uint32_t* __restrict__ patch, uint32_t npatch, const uint8_t* __restrict__ vbt,
     const double* __restrict__ vbv, const double* __restrict__ vol, const double* __restrict__ phi,
-    const double* __restrict__ aon, const double* __restrict__ ano, const double* __restrict__ nof,
-    const double* __restrict__ gradp, const double* __restrict__ body, const double* __restrict__ U, int bounded,
-    const int* __restrict__ flags, int relax, double alpha, double* __restrict__ diag_out,
-    double* __restrict__ rhs_out, int* __restrict__ err) {
+    const double* __restrict__ ano, const double* __restrict__ gradp, const double* __restrict__ body,
+    const double* __restrict__ U, int bounded, const int* __restrict__ flags, int relax, double alpha,
+    double* __restrict__ diag_out, double* __restrict__ rhs_out, int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
     const bool needs = flags[0] && flags[1];
     const size_t n1 = nc;
+    const uint32_t deg = coff[c + 1] - coff[c];
     double diag = 0.0, div = 0.0;
     double rhs[3] = {0.0, 0.0, 0.0}, def[3] = {0.0, 0.0, 0.0};
-    for (uint32_t q = coff[c]; q < coff[c + 1]; ++q) {
-        const uint32_t f = cfs[q];
-        const double F = phi[f];
-        if (neigh[f] >= 0) {
-            const bool own = owner[f] == c;
-            diag += own ? aon[f] : ano[f];
-            div = own ? div + F : div - F;
+    for (uint32_t k = 0; k < deg; ++k) {
+        const size_t idx = size_t(slot_sorted[size_t(k) * nc + c]) * nc + c;
+        const int32_t o = sl.other[idx];
+        if (o == -2) continue;
+        if (o >= 0) {
+            diag += ms.diag[idx];
+            div += ms.div[idx];
             if (needs) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const double t = nof[k * size_t(nf) + f];
-                    def[k] = own ? def[k] + t : def[k] - t;
-                }
+                def[0] += ms.d0[idx];
+                def[1] += ms.d1[idx];
+                def[2] += ms.d2[idx];
             }
         } else {
+            const uint32_t f = sl.face[idx];
             const uint32_t pp = patch[f];
-            if (pp < npatch && kind[pp] == 1) continue;
+            const double F = phi[f];
             if (needs) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) def[k] += nof[k * size_t(nf) + f];
+                def[0] += ms.d0[idx];
+                def[1] += ms.d1[idx];
+                def[2] += ms.d2[idx];
             }
-            diag += aon[f];
+            diag += ms.diag[idx];
             if (pp < npatch && vbt[pp] == 1) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) rhs[k] += ano[f] * vbv[3 * size_t(pp) + k];
+                for (int q = 0; q < 3; ++q) rhs[q] += ano[f] * vbv[3 * size_t(pp) + q];
             } else if (!bounded && F < 0.0) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) def[k] -= F * U[k * n1 + c];
+                for (int q = 0; q < 3; ++q) def[q] -= F * U[q * n1 + c];
             }
             div += F;
         }
@@ -1122,11 +1153,13 @@ __global__ __launch_bounds__(kBlock) void momentum_cell_kernel(
     for (int k = 0; k < 3; ++k) rhs_out[k * n1 + c] = rhs[k];
 }
 
-// CSR values of the momentum matrix (caller numbering), one thread per row:
-// the diagonal from diag; off-diagonal (r, j) = 0 - the coefficient of each
-// face joining r and j in ascending caller face order (ano when r owns the
-// face, else aon). rowmap: internal id of each caller row; nzf: internal face
-// ids; owner, diag, aon, ano: internal numbering.
+// CSR values of the momentum matrix (caller numbering), one thread per
+// caller row r (internal row rowmap[r]): the diagonal from diag; off-diagonal
+// (r, j) = 0 - the coefficient of each face joining r and j in ascending
+// caller face order (ano when r owns the face, else aon). nzf: internal face
+// ids; owner, diag, aon, ano: internal numbering. (Threads over internal rows
+// were measured slower on a randomly numbered mesh: the caller's CSR
+// metadata and the output then scatter instead.)
 __global__ __launch_bounds__(kBlock) void momentum_values_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
                                                                 const uint32_t* __restrict__ cols,
                                                                 const uint32_t* __restrict__ rowmap,
@@ -1307,7 +1340,7 @@ FvMesh::~FvMesh() { release(); }
 void FvMesh::release() {
     for (double** p : {&px, &py, &pz, &fcx, &fcy, &fcz, &fsx, &fsy, &fsz, &ccx, &ccy, &ccz, &vol, &vol_caller,
                        &work_coeff, &work_rc, &work_bsrc, &work_diag, &work_rhs, &work_vals, &work_grad, &work_phi,
-                       &mom_aon, &mom_ano, &mom_nof, &mom_ugrad, &bc_f64, &slot_w, &slot_sx, &slot_sy, &slot_sz,
+                       &mom_aon, &mom_ano, &mom_ugrad, &bc_f64, &mom_sdiag, &mom_sdiv, &mom_sd0, &mom_sd1, &mom_sd2, &slot_w, &slot_sx, &slot_sy, &slot_sz,
                        &wls_a0, &wls_a1, &wls_a2, &wls_fac, &wls_nd, &wls_sig, &wls_V, &wls_cond})
         dfree(*p);
     for (uint32_t** p : {&foff, &fvtx, &owner, &coff, &cfaces, &cfaces_sorted, &patch, &cell_new, &cell_old,
@@ -1317,6 +1350,9 @@ void FvMesh::release() {
     scratch.clear();
     dfree(scratch_u8);
     dfree(slot_other);
+    dfree(face_slot_o);
+    dfree(face_slot_n);
+    dfree(slot_sorted);
     dfree(slot_face);
     dfree(wls_meta);
     dfree(neigh);
@@ -1378,9 +1414,27 @@ void FvMesh::create(int device, uint64_t np_, const double* hx, const double* hy
         });
         cnew = rcm_ordering(g);
     }
-    for (uint32_t c = 0; c < nc; ++c) cold[cnew[c]] = c;
-    // Face order: by internal owner, caller order within an owner.
+    // Keep the caller's numbering when it is about as local as RCM (bandwidth
+    // within a factor of two): then no field is permuted at all.
     {
+        uint64_t bw_caller = 0, bw_rcm = 0;
+        for (uint32_t f = 0; f < nf; ++f) {
+            if (h_neigh[f] < 0) continue;
+            const uint32_t o = h_owner[f], n = uint32_t(h_neigh[f]);
+            bw_caller = std::max<uint64_t>(bw_caller, o > n ? o - n : n - o);
+            const uint32_t a = cnew[o], b = cnew[n];
+            bw_rcm = std::max<uint64_t>(bw_rcm, a > b ? a - b : b - a);
+        }
+        identity_order = bw_caller <= 2 * bw_rcm;
+        if (identity_order) std::iota(cnew.begin(), cnew.end(), 0u);
+    }
+    for (uint32_t c = 0; c < nc; ++c) cold[cnew[c]] = c;
+    // Face order: by internal owner, caller order within an owner (the
+    // caller's order when the cell numbering is kept).
+    if (identity_order) {
+        std::iota(fold.begin(), fold.end(), 0u);
+        std::iota(fnew.begin(), fnew.end(), 0u);
+    } else {
         std::vector<uint32_t> cnt(uint64_t(nc) + 1, 0);
         for (uint32_t f = 0; f < nf; ++f) ++cnt[cnew[h_owner[f]] + 1];
         for (uint32_t c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
@@ -1463,24 +1517,63 @@ double* FvMesh::scr(size_t slot, uint64_t count) {
 }
 
 void FvMesh::cells_in(const double* caller, double* internal, int comps) {
+    if (identity_order) {
+        HIP_OK(hipMemcpyAsync(internal, caller, uint64_t(comps) * nc * 8, hipMemcpyDeviceToDevice, stream));
+        return;
+    }
     for (int k = 0; k < comps; ++k)
         gather_u32idx_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, cell_old, caller + size_t(k) * nc,
                                                                   internal + size_t(k) * nc);
 }
 
 void FvMesh::cells_out(const double* internal, double* caller, int comps) {
+    if (internal == caller) return;
+    if (identity_order) {
+        HIP_OK(hipMemcpyAsync(caller, internal, uint64_t(comps) * nc * 8, hipMemcpyDeviceToDevice, stream));
+        return;
+    }
     for (int k = 0; k < comps; ++k)
         gather_u32idx_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, cell_new, internal + size_t(k) * nc,
                                                                   caller + size_t(k) * nc);
 }
 
 void FvMesh::faces_in(const double* caller, double* internal) {
+    if (identity_order) {
+        HIP_OK(hipMemcpyAsync(internal, caller, uint64_t(nf) * 8, hipMemcpyDeviceToDevice, stream));
+        return;
+    }
     gather_u32idx_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, face_old, caller, internal);
 }
 
 void FvMesh::faces_out(const double* internal, double* caller) {
+    if (internal == caller) return;
+    if (identity_order) {
+        HIP_OK(hipMemcpyAsync(caller, internal, uint64_t(nf) * 8, hipMemcpyDeviceToDevice, stream));
+        return;
+    }
     gather_u32idx_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, face_new, internal, caller);
 }
+
+// Caller-ordered input as an internal array: the caller's own buffer when the
+// numbering is kept, else a permuted copy in scratch slot `slot`.
+const double* FvMesh::cin(const double* caller, size_t slot, int comps) {
+    if (identity_order) return caller;
+    double* b = scr(slot, uint64_t(comps) * nc);
+    cells_in(caller, b, comps);
+    return b;
+}
+const double* FvMesh::fin(const double* caller, size_t slot) {
+    if (identity_order) return caller;
+    double* b = scr(slot, nf);
+    faces_in(caller, b);
+    return b;
+}
+// Where a kernel writes a caller-ordered output (the caller's buffer when the
+// numbering is kept); cells_out / faces_out then move it if needed.
+double* FvMesh::cout_buf(double* caller, size_t slot, int comps) {
+    return identity_order ? caller : scr(slot, uint64_t(comps) * nc);
+}
+double* FvMesh::fout_buf(double* caller, size_t slot) { return identity_order ? caller : scr(slot, nf); }
 
 void FvMesh::check_err(const char* what) {
     int h[4] = {0, 0, 0, 0};
@@ -1534,6 +1627,7 @@ void FvMesh::ensure_slots() {
     if (slot_other) return;
     uint32_t D = 0;
     for (uint32_t c = 0; c < nc; ++c) D = std::max(D, h_coff[c + 1] - h_coff[c]);
+    if (D > 255) throw std::invalid_argument("cells with more than 255 faces are not supported");
     slot_D = D;
     const uint64_t m = uint64_t(D) * nc;
     slot_other = dalloc<int32_t>(m);
@@ -1542,12 +1636,45 @@ void FvMesh::ensure_slots() {
     slot_sx = dalloc<double>(m);
     slot_sy = dalloc<double>(m);
     slot_sz = dalloc<double>(m);
+    // Slot of each face on its owner's and neighbour's side, and each cell's
+    // slots in ascending caller face id (the order of the reference's face
+    // loop accumulations).
+    {
+        std::vector<uint8_t> so(nf, 0), sn(nf, 0), sorted(m, 0);
+        std::vector<uint32_t> cold(nc);
+        for (uint32_t c = 0; c < nc; ++c) cold[h_cell_new[c]] = c;
+        parallel_for(nc, 0, [&](uint64_t i) {
+            const uint32_t c = cold[i];
+            const uint32_t b = h_coff[c], deg = h_coff[c + 1] - b;
+            uint32_t order[256];
+            for (uint32_t q = 0; q < deg && q < 256; ++q) {
+                const uint32_t f = h_cf[b + q];
+                (h_owner[f] == c ? so : sn)[h_face_new[f]] = uint8_t(q);
+                order[q] = q;
+            }
+            std::sort(order, order + std::min<uint32_t>(deg, 256),
+                      [&](uint32_t x, uint32_t y) { return h_cf[b + x] < h_cf[b + y]; });
+            for (uint32_t k = 0; k < deg && k < 256; ++k) sorted[uint64_t(k) * nc + i] = uint8_t(order[k]);
+        });
+        face_slot_o = upload(so.data(), nf);
+        face_slot_n = upload(sn.data(), nf);
+        slot_sorted = upload(sorted.data(), m);
+    }
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     slot_build_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, D, coff, cfaces, owner, neigh, patch, npatch,
                                                            patch_kind, fc, fs, cc, slots_of(*this), err);
     HIP_OK(hipStreamSynchronize(stream));
     check_err("cell stencils: degenerate face distances");
+}
+
+// Per-slot terms written by the face kernels (shared by the momentum and the
+// pressure assembly; scratch within one call).
+void FvMesh::ensure_slot_terms() {
+    ensure_slots();
+    if (mom_sdiag) return;
+    const uint64_t m = uint64_t(slot_D) * nc;
+    for (double** p : {&mom_sdiag, &mom_sdiv, &mom_sd0, &mom_sd1, &mom_sd2}) *p = dalloc<double>(m);
 }
 
 // WLS fit cache for one set of boundary types (fixed value or not, per patch).
@@ -1585,9 +1712,8 @@ void FvMesh::gauss_gradient_internal(const uint8_t* d_bct, const double* d_bcv, 
 }
 
 void FvMesh::gauss_gradient_device(const uint8_t* d_bct, const double* d_bcv, const double* d_p, double* d_grad) {
-    double* p_i = scr(9, nc);
-    double* g_i = scr(8, 3 * uint64_t(nc));
-    cells_in(d_p, p_i, 1);
+    const double* p_i = cin(d_p, 9, 1);
+    double* g_i = cout_buf(d_grad, 8, 3);
     gauss_gradient_internal(d_bct, d_bcv, p_i, g_i);
     cells_out(g_i, d_grad, 3);
 }
@@ -1595,9 +1721,8 @@ void FvMesh::gauss_gradient_device(const uint8_t* d_bct, const double* d_bcv, co
 void FvMesh::mass_flux_device(const uint8_t* d_vbt, const double* d_vbv, double rho, const double* d_U,
                               double* d_phi) {
     const Soa3 fs{fsx, fsy, fsz};
-    double* u_i = scr(0, 3 * uint64_t(nc));
-    double* phi_i = scr(4, nf);
-    cells_in(d_U, u_i, 3);
+    const double* u_i = cin(d_U, 0, 3);
+    double* phi_i = fout_buf(d_phi, 4);
     mass_flux_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind, d_vbt,
                                                           d_vbv, rho, fs, u_i, phi_i);
     faces_out(phi_i, d_phi);
@@ -1726,19 +1851,18 @@ const PressurePlan& FvMesh::pressure_assemble_device(const uint8_t* h_pbt, const
     HIP_OK(hipMemcpy(fixed_face, ff.data(), nf, hipMemcpyHostToDevice));
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
-    double* rau_i = scr(5, 3 * uint64_t(nc));
-    double* phi_i = scr(4, nf);
-    double* p_i = scr(9, nc);
-    cells_in(d_rAU, rau_i, 3);
-    faces_in(d_phi, phi_i);
-    cells_in(d_p, p_i, 1);
+    const double* rau_i = cin(d_rAU, 5, 3);
+    const double* phi_i = fin(d_phi, 4);
+    const double* p_i = cin(d_p, 9, 1);
     gauss_gradient_internal(d_pbt, d_pbv, p_i, work_grad);
+    ensure_slot_terms();
     pressure_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, d_pbt, d_pbv, rho,
                                                               fc, fs, cc, vol, rau_i, p_i, work_grad, work_coeff,
-                                                              work_rc, work_bsrc, err);
-    pressure_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces, cfaces_sorted, owner, neigh, phi_i,
-                                                              work_coeff, work_rc, work_bsrc, fixed_face, work_diag,
-                                                              work_rhs);
+                                                              work_rc, work_bsrc, phi_i, face_slot_o, face_slot_n,
+                                                              mom_sdiv, mom_sd0, mom_sdiag, err);
+    pressure_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, slot_sorted, slots_of(*this), mom_sdiv,
+                                                              mom_sd0, mom_sdiag, work_coeff, work_bsrc, fixed_face,
+                                                              work_diag, work_rhs);
     if (!eliminate) {
         csr_values_kernel<<<grid_for(plan.nnz), kBlock, 0, stream>>>(plan.nnz, pressure_full.nz_row_diag,
                                                                      pressure_full.nzf_off, pressure_full.nzf,
@@ -1765,17 +1889,19 @@ const PressurePlan& FvMesh::diffusion_assemble_device(double gamma, const double
     const PressurePlan& plan = pressure_plan(false, 0);
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
-    double* src_i = scr(3, nc);
-    double* fval_i = scr(7, nf);
-    double* rhs_i = scr(8, nc);
-    if (!scratch_u8) scratch_u8 = dalloc<uint8_t>(nf);
-    cells_in(d_source, src_i, 1);
-    faces_in(d_fval, fval_i);
-    gather_u8_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, face_old, d_fneu, scratch_u8);
+    const double* src_i = cin(d_source, 3, 1);
+    const double* fval_i = fin(d_fval, 7);
+    double* rhs_i = cout_buf(d_rhs, 8, 1);
+    const uint8_t* fneu_i = d_fneu;
+    if (!identity_order) {
+        if (!scratch_u8) scratch_u8 = dalloc<uint8_t>(nf);
+        gather_u8_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, face_old, d_fneu, scratch_u8);
+        fneu_i = scratch_u8;
+    }
     diffusion_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, owner, neigh, gamma, fc, fs, cc, fval_i,
                                                                work_coeff, err);
     diffusion_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, cfaces_sorted, neigh, fs, vol, src_i, fval_i,
-                                                               scratch_u8, work_coeff, work_diag, rhs_i);
+                                                               fneu_i, work_coeff, work_diag, rhs_i);
     csr_values_kernel<<<grid_for(plan.nnz), kBlock, 0, stream>>>(plan.nnz, plan.nz_row_diag, plan.nzf_off, plan.nzf,
                                                                  work_diag, work_coeff, d_vals);
     cells_out(rhs_i, d_rhs, 1);
@@ -1788,9 +1914,9 @@ const PressurePlan& FvMesh::diffusion_assemble_device(double gamma, const double
 
 void FvMesh::ensure_momentum_work() {
     if (mom_aon) return;
+    ensure_slot_terms();
     mom_aon = dalloc<double>(nf);
     mom_ano = dalloc<double>(nf);
-    mom_nof = dalloc<double>(3 * uint64_t(nf));
     mom_ugrad = dalloc<double>(9 * uint64_t(nc));
     mom_flags = dalloc<int>(4);
     const uint32_t npp = std::max<uint32_t>(npatch, 1);
@@ -1830,8 +1956,7 @@ void FvMesh::wls_gradient_device(const uint8_t* h_bt, const double* h_bv, double
     double* d_bv = upload(h_bv, std::max<uint32_t>(npatch, 1));
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
-    double* f_i = scr(0, nc);
-    double* g_i = scr(8, 3 * uint64_t(nc));
+    double* g_i = cout_buf(d_grad, 8, 3);
     try {
         ensure_wls_cache(h_bt, d_bt);
     } catch (...) {
@@ -1839,7 +1964,7 @@ void FvMesh::wls_gradient_device(const uint8_t* h_bt, const double* h_bv, double
         dfree(d_bv);
         throw;
     }
-    cells_in(d_field, f_i, 1);
+    const double* f_i = cin(d_field, 0, 1);
     wls_gradient_kernel<1><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, slots_of(*this), patch, npatch, d_bt, d_bv, 1, fc,
                                                                 fs, cc, vol, stencil_dimension(), limit, wls_of(*this), f_i,
                                                                 g_i, d_fallback, nullptr, err);
@@ -1865,18 +1990,13 @@ const PressurePlan& FvMesh::momentum_assemble_device(const uint8_t* h_vbt, const
     const double* d_vbv = bc_f64;
     ensure_wls_cache(h_bc_u8.data(), d_vbt);
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
-    double* u_i = scr(0, 3 * uint64_t(nc));
-    double* gp_i = scr(1, 3 * uint64_t(nc));
-    double* body_i = d_body ? scr(2, 3 * uint64_t(nc)) : nullptr;
-    double* mu_i = scr(3, nc);
-    double* phi_i = scr(4, nf);
-    double* rhs_i = scr(8, 3 * uint64_t(nc));
-    double* diag_i = scr(9, nc);
-    cells_in(d_U, u_i, 3);
-    cells_in(d_gradp, gp_i, 3);
-    if (d_body) cells_in(d_body, body_i, 3);
-    cells_in(d_mu, mu_i, 1);
-    faces_in(d_phi, phi_i);
+    const double* u_i = cin(d_U, 0, 3);
+    const double* gp_i = cin(d_gradp, 1, 3);
+    const double* body_i = d_body ? cin(d_body, 2, 3) : nullptr;
+    const double* mu_i = cin(d_mu, 3, 1);
+    const double* phi_i = fin(d_phi, 4);
+    double* rhs_i = cout_buf(d_rhs, 8, 3);
+    double* diag_i = cout_buf(d_diag, 9, 1);
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     HIP_OK(hipMemsetAsync(mom_flags, 0, 4 * sizeof(int), stream));
     viscosity_check_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, mu_i, mom_flags, err);
@@ -1885,14 +2005,14 @@ const PressurePlan& FvMesh::momentum_assemble_device(const uint8_t* h_vbt, const
     wls_gradient_kernel<3><<<grid_for(nc), kBlock, 0, stream>>>(nc, coff, slots_of(*this), patch, npatch, d_vbt, d_vbv, 3, fc,
                                                                 fs, cc, vol, stencil_dimension(), 1.e8, wls_of(*this), u_i,
                                                                 mom_ugrad, nullptr, mom_flags, err);
+    const MomSlots ms{mom_sdiag, mom_sdiv, mom_sd0, mom_sd1, mom_sd2};
     momentum_face_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind, d_vbt,
                                                               fc, fs, cc, vol, mu_i, phi_i, mom_ugrad,
-                                                              bounded ? 1 : 0, mom_flags, mom_aon, mom_ano, mom_nof,
-                                                              err);
+                                                              bounded ? 1 : 0, mom_flags, face_slot_o, face_slot_n,
+                                                              mom_aon, mom_ano, ms, err);
     momentum_cell_kernel<<<grid_for(nc), kBlock, 0, stream>>>(
-        nc, nf, coff, cfaces_sorted, owner, neigh, patch, npatch, patch_kind, d_vbt, d_vbv, vol, phi_i, mom_aon,
-        mom_ano, mom_nof, gp_i, body_i, u_i, bounded ? 1 : 0, mom_flags, alpha != 1.0 ? 1 : 0, alpha, diag_i, rhs_i,
-        err);
+        nc, coff, slot_sorted, slots_of(*this), ms, patch, npatch, d_vbt, d_vbv, vol, phi_i, mom_ano, gp_i, body_i,
+        u_i, bounded ? 1 : 0, mom_flags, alpha != 1.0 ? 1 : 0, alpha, diag_i, rhs_i, err);
     momentum_values_kernel<<<grid_for(nc), kBlock, 0, stream>>>(nc, plan.rowp, plan.cols, cell_new, plan.nzf_off,
                                                                 plan.nzf, owner, diag_i, mom_aon, mom_ano, d_vals);
     cells_out(diag_i, d_diag, 1);
@@ -1931,13 +2051,10 @@ void FvMesh::rhie_chow_flux_device(const uint8_t* h_vbt, const double* h_vbv, co
     const Soa3 fc{fcx, fcy, fcz}, fs{fsx, fsy, fsz}, cc{ccx, ccy, ccz};
     HIP_OK(hipMemsetAsync(err, 0, 4 * sizeof(int), stream));
     positive_check_kernel<<<grid_for(3 * uint64_t(nc)), kBlock, 0, stream>>>(3 * uint64_t(nc), d_rau, 53, err);
-    double* h_i = scr(6, 3 * uint64_t(nc));
-    double* rau_i = scr(5, 3 * uint64_t(nc));
-    double* p_i = scr(9, nc);
-    double* phi_i = scr(4, nf);
-    cells_in(d_hbya, h_i, 3);
-    cells_in(d_rau, rau_i, 3);
-    cells_in(d_p, p_i, 1);
+    const double* h_i = cin(d_hbya, 6, 3);
+    const double* rau_i = cin(d_rau, 5, 3);
+    const double* p_i = cin(d_p, 9, 1);
+    double* phi_i = fout_buf(d_phi, 4);
     gauss_gradient_internal(d_pbt, d_pbv, p_i, work_grad);
     rhie_chow_flux_kernel<<<grid_for(nf), kBlock, 0, stream>>>(nf, nc, owner, neigh, patch, npatch, patch_kind,
                                                                d_vbt, d_vbv, d_pbt, d_pbv, rho, fc, fs, cc, vol, h_i,

@@ -37,6 +37,7 @@ struct FvMesh {
     // internal numbering: new_of_old / old_of_new for cells and faces
     uint32_t *cell_new = nullptr, *cell_old = nullptr, *face_new = nullptr, *face_old = nullptr;
     std::vector<uint32_t> h_cell_new, h_face_new;
+    bool identity_order = false;  // the caller's numbering is kept (no permutation)
     int32_t* neigh = nullptr;
     uint32_t* patch = nullptr;
     uint8_t* patch_kind = nullptr;
@@ -53,7 +54,9 @@ struct FvMesh {
     uint8_t* fixed_face = nullptr;
     // momentum side: per-face upwind/diffusion coefficients, tangential
     // fluxes (3 x nf), velocity WLS gradients (3 x 3 x nc), flags
-    double *mom_aon = nullptr, *mom_ano = nullptr, *mom_nof = nullptr, *mom_ugrad = nullptr;
+    double *mom_aon = nullptr, *mom_ano = nullptr, *mom_ugrad = nullptr;
+    // per-slot momentum terms (slot-major): diagonal, signed flux, signed tangential fluxes
+    double *mom_sdiag = nullptr, *mom_sdiv = nullptr, *mom_sd0 = nullptr, *mom_sd1 = nullptr, *mom_sd2 = nullptr;
     int* mom_flags = nullptr;
     // per-patch boundary data of the momentum-side calls (persistent)
     uint8_t* bc_u8 = nullptr;
@@ -67,6 +70,8 @@ struct FvMesh {
     std::vector<uint8_t> h_kind;
     // slot stencils (slot-major, slot_D slots per cell) and the WLS fit cache
     uint32_t slot_D = 0;
+    uint8_t *face_slot_o = nullptr, *face_slot_n = nullptr;  // slot of each face on either side
+    uint8_t* slot_sorted = nullptr;  // per cell, its slots in ascending caller face id
     int32_t* slot_other = nullptr;
     uint32_t* slot_face = nullptr;
     double *slot_w = nullptr, *slot_sx = nullptr, *slot_sy = nullptr, *slot_sz = nullptr;
@@ -121,7 +126,12 @@ struct FvMesh {
     void cells_out(const double* internal, double* caller, int comps);
     void faces_in(const double* caller, double* internal);
     void faces_out(const double* internal, double* caller);
+    const double* cin(const double* caller, size_t slot, int comps);
+    const double* fin(const double* caller, size_t slot);
+    double* cout_buf(double* caller, size_t slot, int comps);
+    double* fout_buf(double* caller, size_t slot);
     void ensure_slots();
+    void ensure_slot_terms();
     void ensure_wls_cache(const uint8_t* h_bt, const uint8_t* d_bt);
     void gauss_gradient_internal(const uint8_t* d_bct, const double* d_bcv, const double* p_i, double* grad_i);
     void check_err(const char* what);
