@@ -221,6 +221,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=120)
     ap.add_argument("--no-momentum", action="store_true", help="skip the momentum-side measurement")
     ap.add_argument("--momentum-repeat", type=int, default=5)
+    ap.add_argument("--momentum-only", choices=["benchmark", "lattice"], default=None,
+                    help="only measure the momentum side on the benchmark mesh, numbered as the benchmark "
+                         "(random cell permutation) or in lattice order; prints one JSON line (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -229,6 +232,22 @@ def main():
 
     import cfdx_dev_amd as cx
     import cfdx_dev_amd.workloads as W
+
+    if args.momentum_only:
+        import ctypes as C
+
+        try:
+            hip = C.CDLL("libamdhip64.so.7")
+        except OSError:
+            hip = C.CDLL("libamdhip64.so")
+        layers = max(1, round(30 * args.n / 256))
+        m = cx.generate_mesh("box", args.n, args.n, args.n, jitter=0.10, permute=args.momentum_only == "benchmark",
+                             seed=5, wall_layers=layers, first_aspect=1000.0)
+        fv = cx.FvMesh(m)
+        out = momentum_measure(fv, m, hip, cx._native.load(), args.momentum_repeat, log)
+        print(json.dumps({"momentum_side": out, "cells": m.n_cells, "faces": m.n_faces,
+                          "numbering": args.momentum_only}), flush=True)
+        return
 
     t0 = time.perf_counter()
     s = W.c5_box(n=args.n)
