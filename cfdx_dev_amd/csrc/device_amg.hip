@@ -765,6 +765,48 @@ __global__ __launch_bounds__(kBlock) void gs_update_kernel(uint32_t n, uint32_t 
     }
 }
 
+// The first Gram-Schmidt update fused with the second pass's products, for
+// m <= kGsFused: w <- w - sum_j h_j V_j, each thread keeping its V_j[i] in its
+// own column of an LDS tile, then the partials that gs_dot_kernel would
+// compute on the new w (V_j . w at j < m, w . w at m), in the same order, so
+// the values are identical while the basis is read once instead of twice.
+constexpr uint32_t kGsFused = 32;
+__global__ __launch_bounds__(kBlock) void gs_update_dot_kernel(uint32_t n, uint32_t m, const double* __restrict__ V,
+                                                              const double* __restrict__ h, double* __restrict__ w,
+                                                              double* __restrict__ partials, uint32_t stride) {
+#pragma clang fp contract(off)
+    __shared__ double red[4];
+    __shared__ double hs[kGsFused];
+    __shared__ double vt[kGsFused][kBlock];
+    for (uint32_t j = threadIdx.x; j < m; j += kBlock) hs[j] = h[j];
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    double acc[kGsFused];
+    double nrm = 0.0;
+#pragma unroll
+    for (uint32_t j = 0; j < kGsFused; ++j) acc[j] = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + t; i < n; i += gridDim.x * kBlock) {
+        double wi = w[i];
+        for (uint32_t j = 0; j < m; ++j) {
+            const double v = V[size_t(j) * n + i];
+            vt[j][t] = v;
+            wi -= hs[j] * v;
+        }
+        w[i] = wi;
+#pragma unroll
+        for (uint32_t j = 0; j < kGsFused; ++j)
+            if (j < m) acc[j] += vt[j][t] * wi;
+        nrm += wi * wi;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kGsFused; ++j) {
+        const double s = block_sum(j < m ? acc[j] : 0.0, red);
+        if (j < m && threadIdx.x == 0) partials[size_t(j) * stride + blockIdx.x] = s;
+    }
+    const double s = block_sum(nrm, red);
+    if (threadIdx.x == 0) partials[size_t(m) * stride + blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(kBlock) void scale_copy_kernel(uint32_t n, const double* __restrict__ w,
                                                            const double* __restrict__ sc_ptr, double* __restrict__ v) {
 #pragma clang fp contract(off)
@@ -1081,8 +1123,17 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
             // h1 = V^T w and |w|^2 (slots 16..), w -= V h1 and |w'|^2 (304).
             gs_dot_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, w, partials_, kGsStride);
             reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 16);
-            gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 16, w, partials_, kGsStride, false);
-            reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, 1, scal_ + 304);
+            // With j + 1 <= kGsFused the first update also yields the second
+            // pass's products (slots 160..160+j) and |w'|^2 (slot 161+j).
+            const bool fused = j + 1 <= kGsFused;
+            const double* wsq = fused ? scal_ + 160 + j + 1 : scal_ + 304;
+            if (fused) {
+                gs_update_dot_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 16, w, partials_, kGsStride);
+                reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 160);
+            } else {
+                gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 16, w, partials_, kGsStride, false);
+                reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, 1, scal_ + 304);
+            }
             HIP_OK(hipMemcpyAsync(h_scal_, scal_, 305 * sizeof(double), hipMemcpyDeviceToHost, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
             const double* hh = h_scal_;
@@ -1092,12 +1143,15 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
                 return finish(2, its, best);
             }
             for (uint32_t i = 0; i <= j; ++i) Hr(i, j) = hh[16 + i];
-            double hn = std::sqrt(hh[304]);
+            double hn = std::sqrt(fused ? hh[160 + j + 1] : hh[304]);
             if (!(hn > kDgks * an)) {
                 // Cancellation: second pass h2 = V^T w (slots 160..), w -= V h2.
                 ++st.reorthogonalisations;
-                gs_dot_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, w, partials_, kGsStride);
-                reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 160);
+                if (!fused) {
+                    gs_dot_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, w, partials_, kGsStride);
+                    reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, j + 2, scal_ + 160);
+                }
+                wsq = scal_ + 304;
                 gs_update_kernel<<<gs, kBlock, 0, stream_>>>(n, j + 1, V, scal_ + 160, w, partials_, kGsStride,
                                                             false);
                 reduce_partials_kernel<<<1, 1024, 0, stream_>>>(partials_, gs, kGsStride, 1, scal_ + 304);
@@ -1115,7 +1169,7 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
             const bool bd = hn <= floor_ && an > 0.0;
             if (hn > 0.0) {
                 // |w| stays in slot 304 as a square; v_{j+1} = w / sqrt(slot).
-                scale_sqrt_copy_kernel<<<g, kBlock, 0, stream_>>>(n, w, scal_ + 304, V + size_t(j + 1) * n);
+                scale_sqrt_copy_kernel<<<g, kBlock, 0, stream_>>>(n, w, wsq, V + size_t(j + 1) * n);
             }
             for (uint32_t i = 0; i < j; ++i) {
                 const double h0 = Hr(i, j), h1 = Hr(i + 1, j);
