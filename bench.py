@@ -213,11 +213,11 @@ def main():
     ap.add_argument("--spmv-repeat", type=int, default=20)
     ap.add_argument("--mixed-smoother", choices=["chebyshev", "jacobi"], default="chebyshev",
                     help="smoother of the FP32 V-cycle inside mixed FGMRES")
-    ap.add_argument("--cheb-degree", type=int, default=4)
-    ap.add_argument("--cheb-ratio", type=float, default=60.0)
+    ap.add_argument("--cheb-degree", type=int, default=5)
+    ap.add_argument("--cheb-ratio", type=float, default=120.0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned stage (value = mixed solve)")
-    ap.add_argument("--train-n", type=int, default=48, help="training boxes are train_n^3")
-    ap.add_argument("--train-systems", type=int, default=12)
+    ap.add_argument("--train-n", type=int, default=96, help="training boxes are train_n^3")
+    ap.add_argument("--train-systems", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=120)
     ap.add_argument("--no-momentum", action="store_true", help="skip the momentum-side measurement")
     ap.add_argument("--momentum-repeat", type=int, default=5)

@@ -450,14 +450,21 @@ __global__ __launch_bounds__(kBlock) void encode_bwd_kernel(uint32_t n, const fl
     }
 }
 
-// grad[p] = sum over tiles (in tile order) of gpart[tile][p].
-__global__ __launch_bounds__(kBlock) void grad_reduce_kernel(uint32_t tiles, const float* __restrict__ gpart,
-                                                            float* __restrict__ grad) {
+// out[y][p] = sum of gpart[b][p] over b in [y * per, min((y + 1) * per, rows))
+// in row order. The parameter gradient is reduced in two such stages with a
+// fixed grouping (per = kGradChunk tiles, then all chunk sums), so the sum is
+// deterministic and the reduction has enough parallelism.
+constexpr uint32_t kGradChunk = 64;
+__global__ __launch_bounds__(kBlock) void grad_reduce_kernel(uint32_t rows, uint32_t per,
+                                                            const float* __restrict__ gpart,
+                                                            float* __restrict__ out) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= uint32_t(kMpnnParams)) return;
+    const uint32_t b0 = blockIdx.y * per;
+    const uint32_t b1 = std::min(rows, b0 + per);
     float acc = 0.0f;
-    for (uint32_t b = 0; b < tiles; ++b) acc += gpart[size_t(b) * kMpnnParams + p];
-    grad[p] = acc;
+    for (uint32_t b = b0; b < b1; ++b) acc += gpart[size_t(b) * kMpnnParams + p];
+    out[size_t(blockIdx.y) * kMpnnParams + p] = acc;
 }
 
 // Adam (Kingma & Ba 2015, Algorithm 1) with bias corrections c1 = 1/(1-b1^t),
@@ -639,7 +646,9 @@ void Mpnn::backward(DeviceAmg& amg, const float* ge, float* grad_dev) {
         gh2_ = mpnn_alloc<float>(uint64_t(cap_) * kHid);
         gm_ = mpnn_alloc<float>(uint64_t(cap_) * kHid);
     }
-    const uint64_t need = uint64_t(tiles_of(cap_)) * kMpnnParams;
+    // tile partials plus one row per chunk of the two-stage reduction
+    const uint64_t cap_tiles = tiles_of(cap_);
+    const uint64_t need = (cap_tiles + (cap_tiles + kGradChunk - 1) / kGradChunk) * kMpnnParams;
     if (gpart_cap_ < need) {
         MPNN_HIP(hipStreamSynchronize(st));
         mpnn_free(gpart_);
@@ -654,7 +663,16 @@ void Mpnn::backward(DeviceAmg& amg, const float* ge, float* grad_dev) {
         mix_t_kernel<<<grid_of(uint64_t(n) * 4), kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, gm_, gh2_, gh_);
     }
     encode_bwd_kernel<<<tiles, kBlock, 0, st>>>(n, last_r_, s_, scal_, gh_, mask_[0], gpart_);
-    grad_reduce_kernel<<<grid_of(kMpnnParams), kBlock, 0, st>>>(tiles, gpart_, grad_dev);
+    const uint32_t chunks = (tiles + kGradChunk - 1) / kGradChunk;
+    if (chunks <= 1) {
+        grad_reduce_kernel<<<grid_of(kMpnnParams), kBlock, 0, st>>>(tiles, tiles, gpart_, grad_dev);
+    } else {
+        // Chunk sums go to the rows after the tiles' rows (the buffer holds
+        // tiles + chunks rows, see the allocation above), then are summed.
+        float* csum = gpart_ + size_t(tiles) * kMpnnParams;
+        grad_reduce_kernel<<<dim3(grid_of(kMpnnParams), chunks), kBlock, 0, st>>>(tiles, kGradChunk, gpart_, csum);
+        grad_reduce_kernel<<<grid_of(kMpnnParams), kBlock, 0, st>>>(chunks, chunks, csum, grad_dev);
+    }
 }
 
 void Mpnn::adam_step(const float* grad_dev, float lr, hipStream_t s) {
