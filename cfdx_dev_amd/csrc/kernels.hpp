@@ -109,7 +109,11 @@ template <>
 struct OrderedSum<float> {
     static constexpr bool value = false;
 };
-constexpr uint32_t kGroupRowNnz = 12;  // FP32: 16 lanes per row above this mean row length
+#ifndef CFDX_GROUP_LANES
+#define CFDX_GROUP_LANES 16
+#endif
+constexpr uint32_t kGroupRowNnz = 12;                // FP32: lanes per row above this mean row length
+constexpr uint32_t kGroupLanes = CFDX_GROUP_LANES;   // power of two, at most 64
 
 template <class T, class Init, class Epi>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __restrict__ blocks, uint32_t nb,
@@ -182,19 +186,21 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
             }
         } else {
             if constexpr (!kOrdered) {
-                // Long rows (FP32 only, no ordering contract): 16 lanes per
-                // row, strided partial sums combined by a shuffle tree, row
-                // sums staged in LDS; then one thread per row applies the
-                // epilogue, so the epilogue's loads are issued once, together.
+                // Long rows (FP32 only, no ordering contract): kGroupLanes
+                // lanes per row, strided partial sums combined by a shuffle
+                // tree, row sums staged in LDS; then one thread per row
+                // applies the epilogue, so the epilogue's loads are issued
+                // once, together.
                 __shared__ T rsum[kMaxRowsPerBlock];
-                const uint32_t g = threadIdx.x >> 4, lane = threadIdx.x & 15;
-                for (uint32_t row0 = r0; row0 < r1; row0 += kBlock / 16) {
+                const uint32_t g = threadIdx.x / kGroupLanes, lane = threadIdx.x % kGroupLanes;
+                for (uint32_t row0 = r0; row0 < r1; row0 += kBlock / kGroupLanes) {
                     const uint32_t row = row0 + g;
                     T s = T(0);
                     if (row < r1)
-                        for (uint32_t k = rowp[row] - k0 + lane; k < rowp[row + 1] - k0; k += 16) s += prod[k];
+                        for (uint32_t k = rowp[row] - k0 + lane; k < rowp[row + 1] - k0; k += kGroupLanes)
+                            s += prod[k];
 #pragma unroll
-                    for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+                    for (int off = int(kGroupLanes) / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kGroupLanes);
                     if (row < r1 && lane == 0) rsum[row - r0] = s;
                 }
                 __syncthreads();
