@@ -53,7 +53,7 @@ HBM_PEAK_GBS = 8000.0
 # passes (FETCH_SIZE, WRITE_SIZE) over this benchmark; FETCH_SIZE is scaled by
 # 2.0, the factor that makes it read the known size of a 1 GiB device copy.
 MEASURED_TRAFFIC = {
-    ("box256_walls30_ar1000_lognormal1.5", "fp32"): {
+    ("box256_walls30_ar1000_lognormal1.5", "fp32", "csr"): {
         "bytes": 1167249711,
         "source": "profiles/spmv_fp32_box256_hbm_traffic.md",
     },
@@ -64,8 +64,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n, nnz, sv, sx):
-    # values + 4 B column per nonzero, 4 B per row offset, x read once, y written once
+def spmv_bytes(n, nnz, sv, sx, fmt=("csr", None, 0)):
+    """Algorithmic bytes of one SpMV in the storage the kernel streams.
+    CSR: values + 4 B column per nonzero, 4 B per row offset. SELL-64: values
+    + 4 B column per stored slot (padding included), 4 B per slice offset.
+    Both: x read once, y written once."""
+    kind, entries, slices = fmt
+    if kind == "sell64":
+        return entries * (sv + 4) + 4 * (slices + 1) + n * sx + n * sx
     return nnz * (sv + 4) + 4 * (n + 1) + n * sx + n * sx
 
 
@@ -386,11 +392,12 @@ def main():
     level_rates = []
     for l in range(min(4, h.num_levels())):
         nl, nnzl, _ = h.level_info(l)
-        rec = {"level": l, "rows": int(nl), "nnz": int(nnzl)}
+        fmt = h.level_spmv_format(l)
+        rec = {"level": l, "rows": int(nl), "nnz": int(nnzl), "format": fmt[0], "stored": fmt[1]}
         for prec, sz in (("fp32", 4), ("fp64", 8)):
             for kind, extra in (("spmv", 0), ("jacobi", 3)):
                 t = h.level_kernel_seconds(l, prec, kind, 10)
-                by = spmv_bytes(nl, nnzl, sz, sz) + extra * nl * sz
+                by = spmv_bytes(nl, nnzl, sz, sz, fmt) + extra * nl * sz
                 rec[f"{prec}_{kind}"] = {"us": round(t * 1e6, 1), "TBps": round(by / t / 1e12, 3)}
         level_rates.append(rec)
     log(f"[bench] per-level kernels: {level_rates}")
@@ -399,8 +406,9 @@ def main():
     log(f"[bench] device copy 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
     t64 = spmv_time(plain, 0, np.float64)
     t32 = spmv_time(mixed, 1, np.float32)
-    b64 = spmv_bytes(s.n, nnz, 8, 8)
-    b32 = spmv_bytes(s.n, nnz, 4, 4)
+    fmt0 = h.level_spmv_format(0)
+    b64 = spmv_bytes(s.n, nnz, 8, 8, fmt0)
+    b32 = spmv_bytes(s.n, nnz, 4, 4, fmt0)
     gbs64 = b64 / t64 / 1e9
     gbs32 = b32 / t32 / 1e9
     log(f"[bench] SpMV FP64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s | FP32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s")
@@ -424,7 +432,7 @@ def main():
         cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.filter)
         log(f"[bench] CPU baseline ({time.perf_counter() - t0:.1f} s): {cpu}")
 
-    wl_traffic = MEASURED_TRAFFIC.get((s.name, "fp32"))
+    wl_traffic = MEASURED_TRAFFIC.get((s.name, "fp32", fmt0[0]))
     learned_info = None
     if net is not None:
         learned_info = {
@@ -499,8 +507,11 @@ def main():
             "frac": round(gbs32 / HBM_PEAK_GBS, 4),
             "traffic": None if wl_traffic is None else wl_traffic["bytes"],
             "traffic_source": None if wl_traffic is None else wl_traffic["source"],
-            "kernel": "spmv_stream_kernel<float,InitZero,EpiStore> (finest level; the FP32 CSR-stream family "
-                      "carries the mixed solve)",
+            "kernel": ("spmv_sell_kernel<float,InitZero,EpiStore> (finest level, SELL-64 storage; the FP32 "
+                       "SELL-64 family carries the mixed solve)" if fmt0[0] == "sell64" else
+                       "spmv_stream_kernel<float,InitZero,EpiStore> (finest level; the FP32 CSR-stream family "
+                       "carries the mixed solve)"),
+            "storage": {"format": fmt0[0], "stored_entries": fmt0[1], "slices": fmt0[2], "nnz": int(nnz)},
             "bytes_per_launch": int(b32),
             "us_per_launch": round(t32 * 1e6, 2),
         },

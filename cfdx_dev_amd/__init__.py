@@ -204,6 +204,16 @@ class NativeAMGPreconditioner:
             raise RuntimeError(self.last_error())
         return v.value
 
+    def level_spmv_format(self, level: int):
+        """(format, stored_entries, slices) of the storage a level's device
+        SpMV streams: "csr" (stored_entries = nnz) or "sell64" (slots of
+        64-row slices, padding included)."""
+        f, e, sl = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        st = self._lib.cfdxhip_amg_level_spmv_format(self._h, level, C.byref(f), C.byref(e), C.byref(sl))
+        if st != 0:
+            raise RuntimeError(f"cfdxhip_amg_level_spmv_format failed with status {st}")
+        return ("csr", "sell64")[f.value], int(e.value), int(sl.value)
+
     def counters(self):
         """(coarse_size, hierarchy_builds, numeric_updates), hypre_amg.h:72,86-87."""
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -477,6 +477,18 @@ cfdxhip_status cfdxhip_amg_level_kernel_seconds(cfdxhip_amg* h, size_t level, ui
     });
 }
 
+cfdxhip_status cfdxhip_amg_level_spmv_format(const cfdxhip_amg* h, size_t level, uint32_t* format,
+                                             uint64_t* stored_entries, uint64_t* slices) {
+    if (!h || !format || !stored_entries || !slices) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready || !h->dev) return CFDXHIP_ERR_NOT_READY;
+    if (level >= h->dev->num_levels()) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    const DevCsr& A = h->dev->level_matrix(level);
+    *format = A.nslices ? 1u : 0u;
+    *stored_entries = A.nslices ? A.sell_entries : A.nnz;
+    *slices = A.nslices;
+    return CFDXHIP_OK;
+}
+
 cfdxhip_status cfdxhip_amg_spmv_device(cfdxhip_amg* h, uint32_t precision, const void* x, void* y, uint32_t repeat,
                                        double* seconds) {
     if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;

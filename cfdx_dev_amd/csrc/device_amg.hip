@@ -91,6 +91,77 @@ std::vector<uint32_t> row_blocks(const std::vector<uint32_t>& rowp) {
     return b;
 }
 
+// Workgroups (and so dot-product partials) of one SpMV launch.
+uint32_t spmv_groups(const DevCsr& M) {
+    if (M.nslices) return (M.nslices + kBlock / kSellRows - 1) / (kBlock / kSellRows);
+    return M.nblocks;
+}
+
+// Padded slots allowed over the CSR entry count before a matrix keeps the
+// CSR-stream SpMV, and the smallest matrix given a SELL copy.
+constexpr double kSellMaxPad = 0.25;
+constexpr uint32_t kSellMinRows = 4096;
+
+// Slot offsets of the SELL-64 layout of a CSR pattern: slice s holds
+// 64 * (longest row of the slice) slots. Returns false (CSR stays the SpMV
+// format) when padding exceeds kSellMaxPad or the slots overflow 32 bits.
+bool sell_offsets(const std::vector<uint32_t>& rowp, std::vector<uint32_t>& soff) {
+    const uint32_t n = uint32_t(rowp.size() - 1);
+    if (n < kSellMinRows) return false;
+    const uint32_t ns = (n + kSellRows - 1) / kSellRows;
+    soff.assign(size_t(ns) + 1, 0);
+    uint64_t total = 0;
+    for (uint32_t sl = 0; sl < ns; ++sl) {
+        const uint32_t r1 = std::min<uint32_t>(n, (sl + 1) * kSellRows);
+        uint32_t w = 0;
+        for (uint32_t r = sl * kSellRows; r < r1; ++r) w = std::max(w, rowp[r + 1] - rowp[r]);
+        total += uint64_t(w) * kSellRows;
+        if (total >= uint64_t(kSellPad)) return false;
+        soff[sl + 1] = uint32_t(total);
+    }
+    return double(total) <= (1.0 + kSellMaxPad) * double(rowp[n]);
+}
+
+// Fills the SELL values of D from its CSR values (padding slots hold 0).
+void sell_values(DevCsr& D, hipStream_t s) {
+    if (!D.nslices) return;
+    const uint32_t slots = uint32_t(D.sell_entries);
+    if (D.s64) {
+        fill_kernel<double><<<grid_for(slots), kBlock, 0, s>>>(slots, 0.0, D.s64);
+        sell_scatter_kernel<double><<<grid_for(D.n), kBlock, 0, s>>>(D.n, D.rowp, D.soff, D.v64, D.s64);
+    }
+    if (D.s32) {
+        fill_kernel<float><<<grid_for(slots), kBlock, 0, s>>>(slots, 0.0f, D.s32);
+        sell_scatter_sorted_kernel<float><<<grid_for(D.n), kBlock, 0, s>>>(D.n, D.rowp, D.cols, D.soff, D.v32,
+                                                                             D.s32);
+    }
+}
+
+// Gives D (CSR already on the device) its SELL-64 copy when the pattern
+// suits it.
+void build_sell(DevCsr& D, const std::vector<uint32_t>& rowp, hipStream_t s) {
+    std::vector<uint32_t> soff;
+    if (!sell_offsets(rowp, soff)) return;
+    D.nslices = uint32_t(soff.size() - 1);
+    D.sell_entries = soff.back();
+    const uint32_t slots = soff.back();
+    D.soff = upload(soff, s);
+    if (D.v64) {
+        D.scols = dalloc<uint32_t>(slots);
+        fill_kernel<uint32_t><<<grid_for(slots), kBlock, 0, s>>>(slots, kSellPad, D.scols);
+        sell_scatter_kernel<uint32_t><<<grid_for(D.n), kBlock, 0, s>>>(D.n, D.rowp, D.soff, D.cols, D.scols);
+        D.s64 = dalloc<double>(slots);
+    }
+    if (D.v32) {
+        D.scols32 = dalloc<uint32_t>(slots);
+        fill_kernel<uint32_t><<<grid_for(slots), kBlock, 0, s>>>(slots, kSellPad, D.scols32);
+        sell_scatter_sorted_kernel<uint32_t><<<grid_for(D.n), kBlock, 0, s>>>(D.n, D.rowp, D.cols, D.soff, D.cols,
+                                                                                D.scols32);
+        D.s32 = dalloc<float>(slots);
+    }
+    sell_values(D, s);
+}
+
 // Relabels a CSR: row i -> rp[i], column j -> cp[j], entry order kept.
 void permute_csr(const HCsr& A, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cp,
                  std::vector<uint32_t>& rowp, std::vector<uint32_t>& cols, std::vector<uint32_t>& src) {
@@ -133,6 +204,12 @@ void DeviceAmg::release() {
         dfree(m.blocks);
         dfree(m.v64);
         dfree(m.v32);
+        dfree(m.soff);
+        dfree(m.scols);
+        dfree(m.scols32);
+        dfree(m.s64);
+        dfree(m.s32);
+        m.nslices = 0;
     };
     for (auto& L : lv_) {
         free_csr(L.A);
@@ -240,6 +317,7 @@ bool DeviceAmg::build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder
             for (size_t k = 0; k < v32.size(); ++k) v32[k] = float(v64[k]);
             D.v32 = upload(v32, stream_);
         }
+        build_sell(D, rowp, stream_);
         HIP_OK(hipStreamSynchronize(stream_));
     };
     for (size_t l = 0; l < L; ++l) {
@@ -306,7 +384,7 @@ bool DeviceAmg::build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder
     }
     const uint32_t n0 = lv_[0].n;
     d_perm0_ = upload(lv_[0].new_of_old, stream_);
-    partial_stride_ = std::max<uint64_t>(lv_[0].A.nblocks, 4096);
+    partial_stride_ = std::max<uint64_t>(spmv_groups(lv_[0].A), 4096);
     partials_ = dalloc<double>(std::max<uint64_t>(partial_stride_ * 8, uint64_t(kGsStride) * 130));
     scal_ = dalloc<double>(kScalars);
     HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_scal_), kScalars * sizeof(double), hipHostMallocDefault));
@@ -340,6 +418,8 @@ bool DeviceAmg::refresh(const HostHierarchy& h, std::string& err) {
             HIP_OK(hipMemcpy(D.A.v32, f.data(), f.size() * 4, hipMemcpyHostToDevice));
             HIP_OK(hipMemcpy(D.inv32, fi.data(), fi.size() * 4, hipMemcpyHostToDevice));
         }
+        sell_values(D.A, stream_);
+        HIP_OK(hipStreamSynchronize(stream_));
     }
     omega_ = h.omega;
     if ((smoother64_ == SMOOTHER_CHEBYSHEV || smoother32_ == SMOOTHER_CHEBYSHEV) &&
@@ -418,8 +498,35 @@ const float* inv_of<float>(const DevLevel& L) {
     return L.inv32;
 }
 
+template <class T>
+static const T* sell_vals_of(const DevCsr& m);
+template <>
+const double* sell_vals_of<double>(const DevCsr& m) {
+    return m.s64;
+}
+template <>
+const float* sell_vals_of<float>(const DevCsr& m) {
+    return m.s32;
+}
+template <class T>
+static const uint32_t* sell_cols_of(const DevCsr& m);
+template <>
+const uint32_t* sell_cols_of<double>(const DevCsr& m) {
+    return m.scols;
+}
+template <>
+const uint32_t* sell_cols_of<float>(const DevCsr& m) {
+    return m.scols32;
+}
+
 template <class T, class Init, class Epi>
 static void launch_spmv(const DevCsr& M, const T* x, Init init, Epi epi, hipStream_t s) {
+    if (M.nslices) {
+        const uint32_t g = spmv_groups(M);
+        spmv_sell_kernel<T, Init, Epi><<<g, kBlock, 0, s>>>(M.n, M.nslices, g, M.soff, sell_cols_of<T>(M),
+                                                            sell_vals_of<T>(M), x, init, epi);
+        return;
+    }
     if (M.nblocks == 0) return;
     spmv_stream_kernel<T, Init, Epi><<<M.nblocks, kBlock, 0, s>>>(M.blocks, M.nblocks, M.rowp, M.cols,
                                                                   vals_of<T>(M), x, init, epi);
@@ -583,7 +690,7 @@ bool DeviceAmg::pcg(const double* b_caller, double* x_caller, uint64_t max_iter,
     const uint32_t g = grid_for(n);
     scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, b_caller, b);
     scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x_caller, x);
-    const uint32_t nbA = L0.A.nblocks;
+    const uint32_t nbA = spmv_groups(L0.A);
     const uint32_t gs = grid_stride(n);
     double* P0 = partials_;
     double* P1 = partials_ + partial_stride_;
@@ -981,7 +1088,7 @@ bool DeviceAmg::fgmres(const double* b_caller, double* x_caller, int restart, ui
     float* z32 = mixed ? need_vec32(work32_, 1, n) : nullptr;
     const uint32_t g = grid_for(n);
     const uint32_t gs = std::min<uint32_t>(grid_stride(n), kGsStride);
-    const uint32_t nbA = L0.A.nblocks;
+    const uint32_t nbA = spmv_groups(L0.A);
     scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, b_caller, b);
     scatter_perm_kernel<double, double><<<g, kBlock, 0, stream_>>>(n, d_perm0_, x_caller, x);
     double* P0 = partials_;

@@ -21,6 +21,14 @@ struct DevCsr {
     uint32_t nblocks = 0;
     double* v64 = nullptr;
     float* v32 = nullptr;
+    // SELL-64 copy used by the SpMV when the slices pad little (nslices > 0).
+    uint32_t nslices = 0;
+    uint64_t sell_entries = 0;  // stored slots, padding included
+    uint32_t* soff = nullptr;   // nslices + 1 slot offsets
+    uint32_t* scols = nullptr;  // FP64 copy: CSR entry order within a row
+    double* s64 = nullptr;
+    uint32_t* scols32 = nullptr;  // FP32 copy: slots sorted by column
+    float* s32 = nullptr;
 };
 
 struct DevLevel {
@@ -134,6 +142,8 @@ public:
     const uint32_t* device_perm0() const { return d_perm0_; }
     // Seconds per launch of a level's SpMV (kind 0: y = A x) or fused Jacobi
     // sweep (kind 1) on the level's own buffers (x = b = 1), HIP events.
+    size_t num_levels() const { return lv_.size(); }
+    const DevCsr& level_matrix(size_t l) const { return lv_[l].A; }
     bool level_kernel_seconds(uint32_t level, uint32_t precision, uint32_t kind, uint32_t repeat, double& sec,
                               std::string& err);
     // y = A x on caller-ordered device vectors; returns seconds per launch.

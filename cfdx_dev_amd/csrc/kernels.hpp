@@ -242,6 +242,107 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const uint32_t* __r
     }
 }
 
+// ------------------------------------------------------------- SELL-64 --
+//
+// Sliced ELL: rows in slices of 64 (one wavefront, one row per lane); inside
+// a slice the q-th entry of every row is stored together, so entry (slice, q,
+// lane) sits at soff[slice] + q * 64 + lane, and a lane adds its slots in
+// order. Rows shorter than the slice's width end in padding slots (column
+// kSellPad, value 0) that the sum skips. Every load of values and columns is
+// a unit-stride 256 B (FP32) or 512 B (FP64) wavefront access.
+// FP64 copies keep each row's CSR entry order in its slots, so rows are
+// summed exactly as the CSR-stream kernel sums them (bit-identical results).
+// FP32 copies sort each row's slots by column: then the gathers of slot q are
+// the q-th neighbours of 64 consecutive rows, which in a bandwidth-reducing
+// ordering lie close together (few cache lines per gather).
+constexpr uint32_t kSellRows = 64;
+constexpr uint32_t kSellPad = 0xFFFFFFFFu;
+constexpr int kSellUnroll = 8;  // slots in flight per lane
+
+template <class T, class Init, class Epi>
+__global__ __launch_bounds__(kBlock) void spmv_sell_kernel(uint32_t n, uint32_t ns, uint32_t nblk,
+                                                           const uint32_t* __restrict__ soff,
+                                                           const uint32_t* __restrict__ cols,
+                                                           const T* __restrict__ vals, const T* __restrict__ x,
+                                                           Init init, Epi epi) {
+#pragma clang fp contract(off)
+    __shared__ double red[4];
+    const uint32_t b = xcd_block(blockIdx.x, nblk);
+    const uint32_t slice = b * (kBlock / kSellRows) + threadIdx.x / kSellRows;
+    const uint32_t lane = threadIdx.x % kSellRows;
+    const uint32_t row = slice * kSellRows + lane;
+    double part = 0.0;
+    if (slice < ns) {
+        typename Epi::Pre pre{};
+        T s = T(0);
+        if (row < n) {
+            pre = epi.load(row);
+            s = init(row);
+        }
+        const uint32_t e = soff[slice + 1];
+        for (uint32_t k = soff[slice] + lane; k < e; k += kSellRows * kSellUnroll) {
+            uint32_t c[kSellUnroll];
+            T v[kSellUnroll];
+#pragma unroll
+            for (int j = 0; j < kSellUnroll; ++j) {
+                const uint32_t kk = k + uint32_t(j) * kSellRows;
+                c[j] = kSellPad;
+                v[j] = T(0);
+                if (kk < e) {
+                    c[j] = __builtin_nontemporal_load(cols + kk);
+                    v[j] = __builtin_nontemporal_load(vals + kk);
+                }
+            }
+            // Unconditional gathers (padding reads x[0]) keep all of them in
+            // flight together; the select leaves s untouched on padding.
+            T xv[kSellUnroll];
+#pragma unroll
+            for (int j = 0; j < kSellUnroll; ++j) xv[j] = x[c[j] == kSellPad ? 0u : c[j]];
+#pragma unroll
+            for (int j = 0; j < kSellUnroll; ++j) s = c[j] == kSellPad ? s : s + v[j] * xv[j];
+        }
+        if (row < n) part = epi.apply(row, s, pre);
+    }
+    if (Epi::kPartial) {
+        const double bs = block_sum(part, red);
+        if (threadIdx.x == 0) epi.store_partial(blockIdx.x, bs);
+    }
+}
+
+// Column-sorted SELL: entry k of row r goes to the slot of its rank among
+// the row's columns (ascending), so slot q of 64 consecutive rows points the
+// same way for every row and the gathers of one slot stay close together.
+// Changes the summation order: FP32 copies only.
+template <class T>
+__global__ __launch_bounds__(kBlock) void sell_scatter_sorted_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                                     const uint32_t* __restrict__ cols,
+                                                                     const uint32_t* __restrict__ soff,
+                                                                     const T* __restrict__ src, T* __restrict__ dst) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t base = soff[r / kSellRows] + r % kSellRows;
+    const uint32_t b = rowp[r], e = rowp[r + 1];
+    for (uint32_t k = b; k < e; ++k) {
+        const uint32_t c = cols[k];
+        uint32_t rank = 0;
+        for (uint32_t k2 = b; k2 < e; ++k2) rank += (cols[k2] < c || (cols[k2] == c && k2 < k)) ? 1u : 0u;
+        dst[base + rank * kSellRows] = src[k];
+    }
+}
+
+// CSR entry k of row r (k-th of the row: q = k - rowp[r]) -> SELL slot
+// soff[r / 64] + q * 64 + r % 64.
+template <class T>
+__global__ __launch_bounds__(kBlock) void sell_scatter_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                              const uint32_t* __restrict__ soff,
+                                                              const T* __restrict__ src, T* __restrict__ dst) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t base = soff[r / kSellRows] + r % kSellRows;
+    const uint32_t b = rowp[r], e = rowp[r + 1];
+    for (uint32_t k = b; k < e; ++k) dst[base + (k - b) * kSellRows] = src[k];
+}
+
 template <class T>
 struct InitZero {
     __device__ T operator()(uint32_t) const { return T(0); }

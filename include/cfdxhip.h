@@ -179,6 +179,12 @@ cfdxhip_status cfdxhip_amg_spmv_device(cfdxhip_amg* h, uint32_t precision, const
  * 1 FP32. No reference counterpart. */
 cfdxhip_status cfdxhip_amg_level_kernel_seconds(cfdxhip_amg* h, size_t level, uint32_t precision, uint32_t kind,
                                                 uint32_t repeat, double* seconds);
+/* Storage the device SpMV of a level's matrix streams: format 0 = CSR
+ * stream (stored_entries = nnz, slices = 0), 1 = SELL-64 (slices of 64 rows,
+ * stored_entries = slots including padding). Measurement only: the byte
+ * counts of a level's kernels follow from it. No reference counterpart. */
+cfdxhip_status cfdxhip_amg_level_spmv_format(const cfdxhip_amg* h, size_t level, uint32_t* format,
+                                             uint64_t* stored_entries, uint64_t* slices);
 
 /* ------------------------------------------------------------ meshes ---- */
 
