@@ -219,8 +219,9 @@ def main():
     ap.add_argument("--spmv-repeat", type=int, default=20)
     ap.add_argument("--mixed-smoother", choices=["chebyshev", "jacobi"], default="chebyshev",
                     help="smoother of the FP32 V-cycle inside mixed FGMRES")
-    ap.add_argument("--cheb-degree", type=int, default=5)
-    ap.add_argument("--cheb-ratio", type=float, default=120.0)
+    ap.add_argument("--cheb-degree", type=int, default=9, help="Chebyshev degree on the finest level")
+    ap.add_argument("--cheb-degree-coarse", type=int, default=1, help="Chebyshev degree below the finest level")
+    ap.add_argument("--cheb-ratio", type=float, default=240.0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned stage (value = mixed solve)")
     ap.add_argument("--train-n", type=int, default=96, help="training boxes are train_n^3")
     ap.add_argument("--train-systems", type=int, default=4)
@@ -269,7 +270,7 @@ def main():
     h = cx.NativeAMGPreconditioner(method="sa", precision="fp32", prolongator_filter=filt,
                                    filter_keeps_aggregate=True, smoother="jacobi",
                                    smoother_fp32=args.mixed_smoother, chebyshev_degree=args.cheb_degree,
-                                   chebyshev_ratio=args.cheb_ratio)
+                                   chebyshev_ratio=args.cheb_ratio, chebyshev_degree_coarse=args.cheb_degree_coarse)
     t0 = time.perf_counter()
     ok = h.setup(s.n, s.rowp, s.cols, s.vals)
     t_setup = time.perf_counter() - t0
@@ -290,7 +291,7 @@ def main():
         t0 = time.perf_counter()
         params, train_meta = train(n=args.train_n, systems=args.train_systems, steps=args.train_steps,
                                    cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
-                                   log=lambda m: log("[train] " + m))
+                                   cheb_degree_coarse=args.cheb_degree_coarse, log=lambda m: log("[train] " + m))
         t_train = time.perf_counter() - t0
         net = cx.LearnedStage(params=params)
         log(f"[bench] learned stage trained in {t_train:.1f} s")
@@ -470,7 +471,8 @@ def main():
             "tolerance": args.tol,
             "solver": ("FGMRES(30) FP64 outer + " + ("[MPNN stage -> " if net is not None else "[") +
                        "FP32 SA V-cycle (" + (
-                           f"Chebyshev degree {args.cheb_degree}, ratio {args.cheb_ratio:g}"
+                           f"Chebyshev degree {args.cheb_degree} finest / {args.cheb_degree_coarse} coarser, "
+                           f"ratio {args.cheb_ratio:g}"
                            if args.mixed_smoother == "chebyshev" else "6+6 damped Jacobi") + ")]"),
             "plain": "PCG FP64 + FP64 SA V-cycle (6+6 damped Jacobi, certified omega), true residual every iteration",
             "learned_stage": net is not None,

@@ -83,7 +83,7 @@ class NativeAMGPreconditioner:
                  host_threads: int = 0, device: int = 0, strength_threshold: float = 0.25,
                  prolongator_filter: float = 0.0, filter_keeps_aggregate: bool = False,
                  smoother: str = "jacobi", smoother_fp32: str = "jacobi", chebyshev_degree: int = 3,
-                 chebyshev_ratio: float = 30.0):
+                 chebyshev_ratio: float = 30.0, chebyshev_degree_coarse: int = 0):
         self._lib = _native.load()
         opt = AmgOptions()
         self._lib.cfdxhip_amg_options_default(C.byref(opt))
@@ -100,6 +100,7 @@ class NativeAMGPreconditioner:
         opt.smoother = SMOOTHERS[smoother]
         opt.smoother_fp32 = SMOOTHERS[smoother_fp32]
         opt.chebyshev_degree = chebyshev_degree
+        opt.chebyshev_degree_coarse = chebyshev_degree_coarse
         opt.chebyshev_ratio = chebyshev_ratio
         h = C.c_void_p()
         st = self._lib.cfdxhip_amg_create(C.byref(opt), C.byref(h))
@@ -220,8 +221,13 @@ class NativeAMGPreconditioner:
         self._lib.cfdxhip_amg_counters(self._h, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
 
-    def set_chebyshev(self, degree: int, ratio: float = 30.0) -> bool:
-        self.last_status = self._lib.cfdxhip_amg_set_chebyshev(self._h, degree, ratio)
+    def set_chebyshev(self, degree: int, ratio: float = 30.0, degree_coarse=None) -> bool:
+        """New Chebyshev degree / ratio on a set-up hierarchy; degree_coarse
+        (levels below the finest, 0 = degree) is kept when None."""
+        if degree_coarse is None:
+            self.last_status = self._lib.cfdxhip_amg_set_chebyshev(self._h, degree, ratio)
+        else:
+            self.last_status = self._lib.cfdxhip_amg_set_chebyshev_levels(self._h, degree, degree_coarse, ratio)
         return self.last_status == 0
 
     def level_lambda_max(self, level: int) -> float:

@@ -38,7 +38,7 @@ class AmgOptions(C.Structure):
         ("smoother", C.c_uint32),
         ("smoother_fp32", C.c_uint32),
         ("chebyshev_degree", C.c_uint32),
-        ("reserved0", C.c_uint32),
+        ("chebyshev_degree_coarse", C.c_uint32),
         ("chebyshev_ratio", C.c_double),
     ]
 
@@ -150,6 +150,7 @@ def load():
     _sig(lib, "cfdxhip_amg_smoother_omega", st, [vp, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_level_lambda_max", st, [vp, C.c_size_t, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_set_chebyshev", st, [vp, C.c_uint32, C.c_double])
+    _sig(lib, "cfdxhip_amg_set_chebyshev_levels", st, [vp, C.c_uint32, C.c_uint32, C.c_double])
     _sig(lib, "cfdxhip_amg_counters", st,
          [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
     _sig(lib, "cfdxhip_capture_read", st, [C.c_char_p, C.POINTER(vp)])

@@ -95,6 +95,7 @@ void configure(cfdxhip_amg* h) {
     c.smoother64 = h->opt.smoother;
     c.smoother32 = h->opt.smoother_fp32;
     c.cheb_degree = h->opt.chebyshev_degree;
+    c.cheb_degree_coarse = h->opt.chebyshev_degree_coarse;
     c.cheb_ratio = h->opt.chebyshev_ratio;
     h->cfg = c;
 }
@@ -192,7 +193,8 @@ cfdxhip_status cfdxhip_amg_create(const cfdxhip_amg_options* options, cfdxhip_am
         !(h->opt.prolongator_filter >= 0.0 && h->opt.prolongator_filter < 1.0) || h->opt.smoother > 1 ||
         h->opt.smoother_fp32 > 1 ||
         ((h->opt.smoother == CFDXHIP_SMOOTHER_CHEBYSHEV || h->opt.smoother_fp32 == CFDXHIP_SMOOTHER_CHEBYSHEV) &&
-         (h->opt.chebyshev_degree < 1 || h->opt.chebyshev_degree > 16 || !(h->opt.chebyshev_ratio > 1.0) ||
+         (h->opt.chebyshev_degree < 1 || h->opt.chebyshev_degree > 16 || h->opt.chebyshev_degree_coarse > 16 ||
+          !(h->opt.chebyshev_ratio > 1.0) ||
           !std::isfinite(h->opt.chebyshev_ratio)))) {
         delete h;
         return CFDXHIP_ERR_INVALID_ARGUMENT;
@@ -434,18 +436,27 @@ cfdxhip_status cfdxhip_amg_level_lambda_max(const cfdxhip_amg* h, size_t level, 
     return CFDXHIP_OK;
 }
 
-cfdxhip_status cfdxhip_amg_set_chebyshev(cfdxhip_amg* h, uint32_t degree, double ratio) {
+cfdxhip_status cfdxhip_amg_set_chebyshev_levels(cfdxhip_amg* h, uint32_t degree, uint32_t degree_coarse,
+                                                double ratio) {
     if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
-    if (degree < 1 || degree > 16 || !(ratio > 1.0) || !std::isfinite(ratio))
-        return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT, "Chebyshev degree must be in [1, 16] and ratio > 1");
+    if (degree < 1 || degree > 16 || degree_coarse > 16 || !(ratio > 1.0) || !std::isfinite(ratio))
+        return fail(h, CFDXHIP_ERR_INVALID_ARGUMENT,
+                    "Chebyshev degree must be in [1, 16], the coarse degree in [0, 16], and ratio > 1");
     if (!h->ready || h->host.lambda_max.empty())
         return fail(h, CFDXHIP_ERR_NOT_READY, "no Chebyshev smoother configured on a set-up hierarchy");
     h->opt.chebyshev_degree = degree;
+    h->opt.chebyshev_degree_coarse = degree_coarse;
     h->opt.chebyshev_ratio = ratio;
     h->cfg.cheb_degree = degree;
+    h->cfg.cheb_degree_coarse = degree_coarse;
     h->cfg.cheb_ratio = ratio;
-    if (h->dev) h->dev->set_chebyshev_params(h->host, degree, ratio);
+    if (h->dev) h->dev->set_chebyshev_params(h->host, degree, degree_coarse, ratio);
     return CFDXHIP_OK;
+}
+
+cfdxhip_status cfdxhip_amg_set_chebyshev(cfdxhip_amg* h, uint32_t degree, double ratio) {
+    if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    return cfdxhip_amg_set_chebyshev_levels(h, degree, h->opt.chebyshev_degree_coarse, ratio);
 }
 
 cfdxhip_status cfdxhip_amg_counters(const cfdxhip_amg* h, uint64_t* coarse_size, uint64_t* hierarchy_builds,

@@ -56,6 +56,7 @@ struct AmgConfig {
     // 1 Jacobi-preconditioned Chebyshev of cheb_degree steps per smoothing.
     uint32_t smoother64 = SMOOTHER_JACOBI, smoother32 = SMOOTHER_JACOBI;
     uint32_t cheb_degree = 3;
+    uint32_t cheb_degree_coarse = 0;  // levels below the finest; 0: cheb_degree
     double cheb_ratio = 30.0;  // upper / lower end of the damped interval
     uint32_t lambda_iterations = 20;
     bool wants_lambda() const { return smoother64 == SMOOTHER_CHEBYSHEV || smoother32 == SMOOTHER_CHEBYSHEV; }

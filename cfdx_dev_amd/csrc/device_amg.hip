@@ -259,6 +259,7 @@ bool DeviceAmg::build(const HostHierarchy& h, const AmgConfig& cfg, bool reorder
     smoother64_ = cfg.smoother64;
     smoother32_ = cfg.smoother32;
     cheb_degree_ = cfg.cheb_degree;
+    cheb_degree_coarse_ = cfg.cheb_degree_coarse;
     cheb_ratio_ = cfg.cheb_ratio;
     if ((smoother64_ == SMOOTHER_CHEBYSHEV || smoother32_ == SMOOTHER_CHEBYSHEV) &&
         h.lambda_max.size() != h.lv.size()) {
@@ -551,8 +552,11 @@ void DeviceAmg::spmv_jacobi_sweeps(DevLevel& L, const T* b, T*& x, uint32_t swee
 void DeviceAmg::set_chebyshev(const HostHierarchy& h) {
     cheb_c1_.assign(h.lambda_max.size(), {});
     cheb_c2_.assign(h.lambda_max.size(), {});
-    for (size_t l = 0; l < h.lambda_max.size(); ++l)
-        chebyshev_coefficients(h.lambda_max[l], cheb_ratio_, cheb_degree_, cheb_c1_[l], cheb_c2_[l]);
+    cheb_deg_.assign(h.lambda_max.size(), cheb_degree_);
+    for (size_t l = 0; l < h.lambda_max.size(); ++l) {
+        if (l > 0 && cheb_degree_coarse_ > 0) cheb_deg_[l] = cheb_degree_coarse_;
+        chebyshev_coefficients(h.lambda_max[l], cheb_ratio_, cheb_deg_[l], cheb_c1_[l], cheb_c2_[l]);
+    }
 }
 
 template <class T>
@@ -573,12 +577,13 @@ void DeviceAmg::chebyshev_sweeps(uint32_t l, const T* b, T*& x, bool from_zero) 
     T* d = d_of<T>(L);
     const auto& c1 = cheb_c1_[l];
     const auto& c2 = cheb_c2_[l];
+    const uint32_t deg = cheb_deg_[l];
     uint32_t k = 0;
-    if (from_zero && cheb_degree_ > 0) {
+    if (from_zero && deg > 0) {
         cheb_first_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, b, inv_of<T>(L), T(c2[0]), d, x);
         k = 1;
     }
-    for (; k < cheb_degree_; ++k) {
+    for (; k < deg; ++k) {
         T* xn = (x == xs[0]) ? xs[1] : xs[0];
         if (k == 0)
             launch_spmv<T>(L.A, x, InitZero<T>{},
@@ -614,7 +619,7 @@ bool DeviceAmg::vcycle_t(uint32_t l, const T* b, T*& x) {
         return true;
     }
     const bool cheb = (sizeof(T) == 8 ? smoother64_ : smoother32_) == SMOOTHER_CHEBYSHEV;
-    if (cheb ? cheb_degree_ > 0 : pre_ > 0) {
+    if (cheb ? cheb_deg_[l] > 0 : pre_ > 0) {
         smooth<T>(l, b, x, pre_, true);
     } else {
         fill_kernel<T><<<grid_for(L.n), kBlock, 0, stream_>>>(L.n, T(0), x);

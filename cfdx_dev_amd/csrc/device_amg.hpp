@@ -151,8 +151,9 @@ public:
 
     // Changes the Chebyshev degree / interval ratio of a built hierarchy
     // (coefficients only; the eigenvalue estimates are kept).
-    void set_chebyshev_params(const HostHierarchy& h, uint32_t degree, double ratio) {
+    void set_chebyshev_params(const HostHierarchy& h, uint32_t degree, uint32_t degree_coarse, double ratio) {
         cheb_degree_ = degree;
+        cheb_degree_coarse_ = degree_coarse;
         cheb_ratio_ = ratio;
         set_chebyshev(h);
     }
@@ -181,8 +182,10 @@ private:
     uint32_t pre_ = 6, post_ = 6;
     uint32_t smoother64_ = SMOOTHER_JACOBI, smoother32_ = SMOOTHER_JACOBI;
     uint32_t cheb_degree_ = 3;
+    uint32_t cheb_degree_coarse_ = 0;  // levels below the finest; 0: cheb_degree_
     double cheb_ratio_ = 30.0;
     std::vector<std::vector<double>> cheb_c1_, cheb_c2_;  // per level
+    std::vector<uint32_t> cheb_deg_;                       // per level
     bool has64_ = false, has32_ = false;
     // Called by pcg with (iteration, x in the solve ordering) after every
     // update of x, with iteration 0 for the initial guess and UINT64_MAX for

@@ -23,8 +23,8 @@ def replay(path, solver="cg", method="sa", prolongator_filter=0.0, keep_aggregat
     mixed = solver == "mixed"
     h = NativeAMGPreconditioner(method=method, precision="fp32" if mixed else "fp64",
                                 prolongator_filter=prolongator_filter, filter_keeps_aggregate=keep_aggregate,
-                                smoother_fp32="chebyshev" if mixed else "jacobi", chebyshev_degree=5,
-                                chebyshev_ratio=120.0, device=device)
+                                smoother_fp32="chebyshev" if mixed else "jacobi", chebyshev_degree=9,
+                                chebyshev_degree_coarse=1, chebyshev_ratio=240.0, device=device)
     t0 = time.perf_counter()
     if not h.setup(cap.n, cap.rowp, cap.cols, cap.vals):
         return dict(file=str(path), status="setup_failed", error=h.last_error())

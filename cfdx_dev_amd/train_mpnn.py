@@ -32,17 +32,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "weights", "mpnn_box.npy")
 
 
-def amg_for(s, device=0, cheb_degree=3, cheb_ratio=30.0):
+def amg_for(s, device=0, cheb_degree=3, cheb_ratio=30.0, cheb_degree_coarse=0):
     h = NativeAMGPreconditioner(precision="fp32", prolongator_filter=0.2, filter_keeps_aggregate=True,
                                 smoother_fp32="chebyshev", chebyshev_degree=cheb_degree, chebyshev_ratio=cheb_ratio,
-                                device=device)
+                                chebyshev_degree_coarse=cheb_degree_coarse, device=device)
     if not h.setup(s.n, s.rowp, s.cols, s.vals):
         raise RuntimeError(h.last_error())
     return h
 
 
 def train(n=48, systems=12, steps=120, lr=2e-3, seed=0, device=0, tol=1e-8, cheb_degree=3, cheb_ratio=30.0,
-          log=print):
+          log=print, cheb_degree_coarse=0):
     """Trains against the FP32 V-cycle it will run with (the Chebyshev degree
     and ratio matter: a stage trained for one V-cycle can hurt another)."""
     net = LearnedStage(device=device, params=LearnedStage.initial_params(seed))
@@ -50,7 +50,7 @@ def train(n=48, systems=12, steps=120, lr=2e-3, seed=0, device=0, tol=1e-8, cheb
     t0 = time.perf_counter()
     for k in range(systems):
         s = W.c5_box(n=n, seed=1000 + k, device=device)
-        h = amg_for(s, device, cheb_degree, cheb_ratio)
+        h = amg_for(s, device, cheb_degree, cheb_ratio, cheb_degree_coarse)
         x = np.zeros(s.n)
         plain = h.solve_fgmres(s.b, x, 30, 1000, tol)
         if plain.status == 0 and plain.iterations > 0:
@@ -73,13 +73,13 @@ def train(n=48, systems=12, steps=120, lr=2e-3, seed=0, device=0, tol=1e-8, cheb
     trained = [r for r in history if "skipped" not in r]
     meta = dict(n=n, systems=systems, systems_trained=len(trained), pairs=int(sum(r["pairs"] for r in trained)),
                 steps=steps, lr=lr, seed=seed, pair_iterations=list(PAIR_ITERATIONS),
-                cheb_degree=cheb_degree, cheb_ratio=cheb_ratio,
+                cheb_degree=cheb_degree, cheb_ratio=cheb_ratio, cheb_degree_coarse=cheb_degree_coarse,
                 plain_rate=float(np.mean(rates)) if rates else 0.0, history=history)
     return net.get_params(), meta
 
 
 def validate(params, n=64, systems=80, seed0=5000, device=0, tol=1e-8, cheb_degree=3, cheb_ratio=30.0,
-             log=print):
+             log=print, cheb_degree_coarse=0):
     """Mixed FGMRES(30) + one FP32 V-cycle on unseen boxes (seeds seed0..),
     without the stage and with it (fallback off): per-system iterations and
     solve seconds, and their means."""
@@ -88,7 +88,7 @@ def validate(params, n=64, systems=80, seed0=5000, device=0, tol=1e-8, cheb_degr
     rows = []
     for k in range(systems):
         s = W.c5_box(n=n, seed=seed0 + k, device=device)
-        h = amg_for(s, device, cheb_degree, cheb_ratio)
+        h = amg_for(s, device, cheb_degree, cheb_ratio, cheb_degree_coarse)
         x = np.zeros(s.n)
         plain = h.solve_fgmres(s.b, x, 30, 1000, tol)
         x = np.zeros(s.n)
@@ -133,13 +133,16 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cheb-degree", type=int, default=3)
     ap.add_argument("--cheb-ratio", type=float, default=30.0)
+    ap.add_argument("--cheb-degree-coarse", type=int, default=0, help="levels below the finest (0: --cheb-degree)")
     ap.add_argument("--validate-n", type=int, default=0, help="validate on unseen boxes of this size (0: skip)")
     ap.add_argument("--validate-systems", type=int, default=80)
     ap.add_argument("--out", default=DEFAULT_OUT)
     a = ap.parse_args()
-    params, meta = train(a.n, a.systems, a.steps, a.lr, a.seed, cheb_degree=a.cheb_degree, cheb_ratio=a.cheb_ratio)
+    params, meta = train(a.n, a.systems, a.steps, a.lr, a.seed, cheb_degree=a.cheb_degree, cheb_ratio=a.cheb_ratio,
+                         cheb_degree_coarse=a.cheb_degree_coarse)
     if a.validate_n:
-        v = validate(params, a.validate_n, a.validate_systems, cheb_degree=a.cheb_degree, cheb_ratio=a.cheb_ratio)
+        v = validate(params, a.validate_n, a.validate_systems, cheb_degree=a.cheb_degree, cheb_ratio=a.cheb_ratio,
+                     cheb_degree_coarse=a.cheb_degree_coarse)
         meta["validation"] = {k: val for k, val in v.items() if k != "rows"}
         meta["validation_rows"] = v["rows"]
         print(json.dumps(meta["validation"]))

@@ -105,7 +105,8 @@ typedef struct cfdxhip_amg_options {
     uint32_t smoother;                     /* CFDXHIP_SMOOTHER_* of FP64 V-cycles (default JACOBI) */
     uint32_t smoother_fp32;                /* CFDXHIP_SMOOTHER_* of FP32 V-cycles (default JACOBI) */
     uint32_t chebyshev_degree;             /* Chebyshev steps per pre/post smoothing (default 3) */
-    uint32_t reserved0;
+    uint32_t chebyshev_degree_coarse;      /* steps on levels below the finest; 0: chebyshev_degree
+                                              (default 0) */
     double chebyshev_ratio;                /* upper / lower end of the damped interval (default 30) */
 } cfdxhip_amg_options;
 
@@ -161,9 +162,13 @@ cfdxhip_status cfdxhip_amg_counters(const cfdxhip_amg* h, uint64_t* coarse_size,
  * smoother is configured. No reference counterpart. */
 cfdxhip_status cfdxhip_amg_level_lambda_max(const cfdxhip_amg* h, size_t level, double* lambda_max);
 /* Changes the Chebyshev degree and interval ratio of a set-up hierarchy that
- * uses a Chebyshev smoother (coefficients only, no new setup). No reference
- * counterpart. */
+ * uses a Chebyshev smoother (coefficients only, no new setup); the coarse
+ * levels' degree is kept. No reference counterpart. */
 cfdxhip_status cfdxhip_amg_set_chebyshev(cfdxhip_amg* h, uint32_t degree, double ratio);
+/* As cfdxhip_amg_set_chebyshev, with the degree of the levels below the
+ * finest (0: the finest level's degree). No reference counterpart. */
+cfdxhip_status cfdxhip_amg_set_chebyshev_levels(cfdxhip_amg* h, uint32_t degree, uint32_t degree_coarse,
+                                                double ratio);
 /* Seconds spent in the last setup: host construction, device upload. */
 cfdxhip_status cfdxhip_amg_setup_seconds(const cfdxhip_amg* h, double* host_seconds, double* device_seconds);
 

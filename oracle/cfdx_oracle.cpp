@@ -722,6 +722,7 @@ struct Amg {
     // inside the V-cycle): 0 damped Jacobi, 1 Chebyshev.
     int smoother = 0;
     size_t cheb_degree = 3;
+    size_t cheb_degree_coarse = 0;  // levels below the finest; 0: cheb_degree
     double cheb_ratio = 30.0;
     std::vector<double> lambda;  // per-level lambda_max(D^-1 A) estimate
     std::vector<Level> lv;
@@ -812,6 +813,7 @@ void cheb_coeffs(double lambda, double ratio, size_t degree, std::vector<double>
     const double theta = (hi + lo) / 2.0, delta = (hi - lo) / 2.0, sigma = theta / delta;
     keep.assign(degree, 0.0);
     scale.assign(degree, 0.0);
+    if (degree == 0) return;
     scale[0] = 1.0 / theta;
     double rho = 1.0 / sigma;
     for (size_t k = 1; k < degree; ++k) {
@@ -929,13 +931,15 @@ bool amg_update(Amg& h, const double* vals) {
     return true;
 }
 
-// Chebyshev smoothing of degree h.cheb_degree (this build's variant).
+// Chebyshev smoothing (this build's variant): h.cheb_degree steps on the
+// finest level, h.cheb_degree_coarse (when nonzero) on the levels below.
 bool smooth_cheb(const Amg& h, size_t l, const std::vector<double>& r, std::vector<double>& x) {
     const Level& L = h.lv[l];
+    const size_t degree = (l > 0 && h.cheb_degree_coarse > 0) ? h.cheb_degree_coarse : h.cheb_degree;
     std::vector<double> keep, scale;
-    cheb_coeffs(h.lambda[l], h.cheb_ratio, h.cheb_degree, keep, scale);
+    cheb_coeffs(h.lambda[l], h.cheb_ratio, degree, keep, scale);
     std::vector<double> Ax(r.size()), d(r.size(), 0.0);
-    for (size_t k = 0; k < h.cheb_degree; ++k) {
+    for (size_t k = 0; k < degree; ++k) {
         matvec(L.A, x.data(), Ax.data());
         for (size_t i = 0; i < r.size(); ++i) {
             const double delta = L.inv_diag[i] * (r[i] - Ax[i]);
@@ -1427,6 +1431,9 @@ void oracle_amg_smoother(void* hp, int kind, int degree, double ratio) {
     h->smoother = kind;
     h->cheb_degree = size_t(degree);
     h->cheb_ratio = ratio;
+}
+void oracle_amg_smoother_coarse(void* hp, int degree_coarse) {
+    static_cast<Amg*>(hp)->cheb_degree_coarse = size_t(degree_coarse);
 }
 double oracle_amg_lambda_max(void* hp, size_t l) {
     Amg* h = static_cast<Amg*>(hp);

@@ -202,7 +202,8 @@ class Oracle:
 
     # -------------------------------------------------------------- AMG --
     def amg(self, method="sa", policy="balanced", require_certificate=True, theta=0.25, prolongator_filter=0.0,
-            filter_keeps_aggregate=False, smoother="jacobi", chebyshev_degree=3, chebyshev_ratio=30.0):
+            filter_keeps_aggregate=False, smoother="jacobi", chebyshev_degree=3, chebyshev_ratio=30.0,
+            chebyshev_degree_coarse=0):
         a = OracleAmg(self, method, policy, require_certificate)
         getattr(self.lib, self.p + "amg_configure")(a.h, theta, prolongator_filter)
         if filter_keeps_aggregate:
@@ -215,6 +216,9 @@ class Oracle:
                 raise ValueError("the reference V-cycle has no Chebyshev smoother")
             self.lib.oracle_amg_smoother.argtypes = [vp, C.c_int, C.c_int, C.c_double]
             self.lib.oracle_amg_smoother(a.h, {"chebyshev": 1}[smoother], chebyshev_degree, chebyshev_ratio)
+            if chebyshev_degree_coarse:
+                self.lib.oracle_amg_smoother_coarse.argtypes = [vp, C.c_int]
+                self.lib.oracle_amg_smoother_coarse(a.h, chebyshev_degree_coarse)
         return a
 
     # ------------------------------------------------------- momentum side --

@@ -7,7 +7,9 @@ Pinned without the reference:
     form (I - E) A^-1 r, E = p(D^-1 A) (I - P Ac^-1 P^T A) p(D^-1 A);
   * the lambda_max estimate (20 Lanczos steps) as a lower bound within 3 %
     of the true largest eigenvalue of D^-1 A;
-  * the product's host estimate equal to the oracle's, bit for bit.
+  * the product's host estimate equal to the oracle's, bit for bit;
+  * a four-level V-cycle with a separate degree below the finest level
+    against its recursive closed form.
 """
 import numpy as np
 import pytest
@@ -80,6 +82,60 @@ def test_two_level_vcycle_closed_form(oracle):
         assert np.linalg.norm(z - expect) <= 1e-10 * np.linalg.norm(expect)
 
 
+def _dense_level(a, l):
+    n = a.level_info(l)[0]
+    rp, cols, vals = a.level_matrix(l)
+    return csr_to_dense(rp, cols, vals, n)
+
+
+def _vcycle_operator(a, l, degrees, ratio):
+    """Matrix M_l of the V-cycle from level l down (z = M_l r): the coarsest
+    level (<= 16 rows) solved exactly; above it
+    M_l = (I - E_l) A_l^-1, E_l = S_l (I - P_l M_{l+1} P_l^T A_l) S_l,
+    S_l = p_l(D_l^-1 A_l) with p_l the scaled Chebyshev polynomial of degree
+    degrees[l] on [1.1 lambda_l / ratio, 1.1 lambda_l]."""
+    A = _dense_level(a, l)
+    n = A.shape[0]
+    if l + 1 == a.num_levels():
+        assert n <= 16
+        return np.linalg.inv(A)
+    _, prp, pc, pv = a.level_transfer(l)
+    nc = a.level_info(l + 1)[0]
+    P = np.zeros((n, nc))
+    for i in range(n):
+        P[i, pc[prp[i]:prp[i + 1]]] = pv[prp[i]:prp[i + 1]]
+    Mc = _vcycle_operator(a, l + 1, degrees, ratio)
+    d = np.diag(A)
+    w, V = np.linalg.eigh(A / np.sqrt(np.outer(d, d)))
+    hi = 1.1 * a.lambda_max(l)
+    S = np.diag(d ** -0.5) @ (V @ np.diag(cheb_poly(w, hi, hi / ratio, degrees[l])) @ V.T) @ np.diag(d ** 0.5)
+    E = S @ (np.eye(n) - P @ Mc @ P.T @ A) @ S
+    return (np.eye(n) - E) @ np.linalg.inv(A)
+
+
+@pytest.mark.parametrize("degree,coarse", [(3, 1), (5, 2), (9, 1)])
+def test_coarse_degree_vcycle_closed_form(oracle, degree, coarse):
+    """A four-level V-cycle with `degree` Chebyshev steps on the finest level
+    and `coarse` on the two levels below it, against the recursive closed
+    form (a degree applied to the wrong level changes the operator)."""
+    s = pressure_system(oracle, box_mesh(8, 8, 6, jitter=0.15, seed=3), sigma=1.0, seed=3)
+    ratio = 30.0
+    a = oracle.amg("sa", smoother="chebyshev", chebyshev_degree=degree, chebyshev_ratio=ratio,
+                   chebyshev_degree_coarse=coarse)
+    assert a.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    L = a.num_levels()
+    assert L == 4 and a.level_info(L - 1)[0] <= 16
+    M = _vcycle_operator(a, 0, [degree] + [coarse] * (L - 1), ratio)
+    M_uniform = _vcycle_operator(a, 0, [degree] * L, ratio)
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        r = rng.normal(size=s["n"])
+        z = a.apply(r)
+        expect = M @ r
+        assert np.linalg.norm(z - expect) <= 1e-10 * np.linalg.norm(expect)
+        assert np.linalg.norm(z - M_uniform @ r) > 1e-6 * np.linalg.norm(expect)
+
+
 def test_lambda_estimate_bounds(oracle):
     s = pressure_system(oracle, box_mesh(8, 8, 6, seed=4), sigma=1.0, seed=4)
     a = oracle.amg("sa", smoother="chebyshev")
@@ -113,6 +169,8 @@ def test_invalid_chebyshev_options():
         cx.NativeAMGPreconditioner(device=-1, smoother="chebyshev", chebyshev_degree=0)
     with pytest.raises(RuntimeError):
         cx.NativeAMGPreconditioner(device=-1, smoother_fp32="chebyshev", chebyshev_ratio=1.0)
+    with pytest.raises(RuntimeError):
+        cx.NativeAMGPreconditioner(device=-1, smoother_fp32="chebyshev", chebyshev_degree_coarse=17)
     h = cx.NativeAMGPreconditioner(device=-1)
     with pytest.raises(RuntimeError):
         h.level_lambda_max(0)

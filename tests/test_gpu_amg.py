@@ -133,6 +133,52 @@ def test_chebyshev_vcycle_bit_identical(oracle, dims, sigma, degree):
         assert np.array_equal(zo, zp), f"max |dz| = {np.abs(zo - zp).max()}"
 
 
+@pytest.mark.parametrize("degree,coarse,ratio", [(9, 1, 240.0), (4, 2, 30.0)])
+def test_chebyshev_coarse_degree_vcycle_bit_identical(oracle, degree, coarse, ratio):
+    """Separate Chebyshev degree below the finest level: the FP64 V-cycle
+    equals the oracle's bit for bit, set at construction and after setup."""
+    s = pressure_system(oracle, box_mesh(20, 20, 20, jitter=0.15, seed=8), sigma=1.5, seed=8)
+    ao = oracle.amg("sa", smoother="chebyshev", chebyshev_degree=degree, chebyshev_ratio=ratio,
+                    chebyshev_degree_coarse=coarse)
+    assert ao.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    ap = cx.NativeAMGPreconditioner(smoother="chebyshev", chebyshev_degree=degree, chebyshev_ratio=ratio,
+                                    chebyshev_degree_coarse=coarse)
+    assert ap.setup(s["n"], s["rowp"], s["cols"], s["vals"]), ap.last_error()
+    aq = cx.NativeAMGPreconditioner(smoother="chebyshev", chebyshev_degree=3)
+    assert aq.setup(s["n"], s["rowp"], s["cols"], s["vals"]), aq.last_error()
+    assert aq.set_chebyshev(degree, ratio, degree_coarse=coarse), aq.last_error()
+    rng = np.random.default_rng(6)
+    for _ in range(2):
+        r = rng.normal(size=s["n"])
+        zo = ao.apply(r)
+        for h in (ap, aq):
+            zp = h.apply(r)
+            assert zp is not None, h.last_error()
+            assert np.array_equal(zo, zp), f"max |dz| = {np.abs(zo - zp).max()}"
+
+
+def test_mixed_fgmres_coarse_degree_matches_oracle(oracle):
+    """The mixed solve's smoother setting (Chebyshev degree 9 on the finest
+    level, 1 below, ratio 240, FP32 hierarchy): FGMRES reaches the FP64
+    true-residual tolerance and the oracle's PCG solution."""
+    s = pressure_system(oracle, box_mesh(24, 20, 18, jitter=0.15, seed=10), sigma=1.5, seed=10)
+    ao = oracle.amg("sa")
+    assert ao.setup(s["n"], s["rowp"], s["cols"], s["vals"])
+    tol = 1e-12
+    xo = np.zeros(s["n"])
+    assert ao.pcg(s["b"], xo, 500, tol).status == 0
+    ap = cx.NativeAMGPreconditioner(precision="fp32", smoother_fp32="chebyshev", chebyshev_degree=9,
+                                    chebyshev_degree_coarse=1, chebyshev_ratio=240.0)
+    assert ap.setup(s["n"], s["rowp"], s["cols"], s["vals"]), ap.last_error()
+    xp = np.zeros(s["n"])
+    rp = ap.solve_fgmres(s["b"], xp, 30, 500, tol)
+    assert rp.status == 0, ap.last_error()
+    y = oracle.spmv(s["rowp"], s["cols"], s["vals"], xp)
+    assert np.linalg.norm(s["b"] - y) / np.linalg.norm(s["b"]) <= 1.01 * tol
+    rel = np.linalg.norm(xp - xo) / np.linalg.norm(xo)
+    assert rel <= 1e-10, rel
+
+
 def test_chebyshev_fp32_vcycle_close_to_fp64(oracle):
     s = pressure_system(oracle, box_mesh(20, 18, 16, jitter=0.15, seed=9), sigma=1.5, seed=9)
     ao = oracle.amg("sa", smoother="chebyshev")
