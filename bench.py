@@ -53,8 +53,8 @@ HBM_PEAK_GBS = 8000.0
 # passes (FETCH_SIZE, WRITE_SIZE) over this benchmark; FETCH_SIZE is scaled by
 # 2.0, the factor that makes it read the known size of a 1 GiB device copy.
 MEASURED_TRAFFIC = {
-    ("box256_walls30_ar1000_lognormal1.5", "fp32", "csr"): {
-        "bytes": 1167249711,
+    ("box256_walls30_ar1000_lognormal1.5", "fp32", "sell64"): {
+        "bytes": 1076320129,
         "source": "profiles/spmv_fp32_box256_hbm_traffic.md",
     },
 }
