@@ -53,7 +53,8 @@ HBM_PEAK_GBS = 8000.0
 # passes (FETCH_SIZE, WRITE_SIZE) over this benchmark; FETCH_SIZE is scaled by
 # 2.0, the factor that makes it read the known size of a 1 GiB device copy.
 MEASURED_TRAFFIC = {
-    ("box256_walls30_ar1000_lognormal1.5", "fp32", "sell64"): {
+    # keyed by (workload, precision, storage, matrix bytes of the kernel measured)
+    ("box256_walls30_ar1000_lognormal1.5", "fp32", "sell64", 939820036): {
         "bytes": 1076320129,
         "source": "profiles/spmv_fp32_box256_hbm_traffic.md",
     },
@@ -64,15 +65,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n, nnz, sv, sx, fmt=("csr", None, 0)):
-    """Algorithmic bytes of one SpMV in the storage the kernel streams.
-    CSR: values + 4 B column per nonzero, 4 B per row offset. SELL-64: values
-    + 4 B column per stored slot (padding included), 4 B per slice offset.
-    Both: x read once, y written once."""
-    kind, entries, slices = fmt
-    if kind == "sell64":
-        return entries * (sv + 4) + 4 * (slices + 1) + n * sx + n * sx
-    return nnz * (sv + 4) + 4 * (n + 1) + n * sx + n * sx
+def spmv_bytes(n, matrix_bytes, sx):
+    """Algorithmic bytes of one SpMV: the matrix stream in the storage the
+    kernel reads (the library's count: CSR = values + 4 B column per nonzero +
+    4 B per row offset; SELL-64 = values + 4 B column per stored slot, padding
+    included, 4 B per slice offset), x read once, y written once."""
+    return matrix_bytes + n * sx + n * sx
 
 
 def cpu_model():
@@ -396,9 +394,11 @@ def main():
         fmt = h.level_spmv_format(l)
         rec = {"level": l, "rows": int(nl), "nnz": int(nnzl), "format": fmt[0], "stored": fmt[1]}
         for prec, sz in (("fp32", 4), ("fp64", 8)):
+            mb = h.level_spmv_bytes(l, prec)
+            rec[f"{prec}_matrix_bytes"] = mb
             for kind, extra in (("spmv", 0), ("jacobi", 3)):
                 t = h.level_kernel_seconds(l, prec, kind, 10)
-                by = spmv_bytes(nl, nnzl, sz, sz, fmt) + extra * nl * sz
+                by = spmv_bytes(nl, mb, sz) + extra * nl * sz
                 rec[f"{prec}_{kind}"] = {"us": round(t * 1e6, 1), "TBps": round(by / t / 1e12, 3)}
         level_rates.append(rec)
     log(f"[bench] per-level kernels: {level_rates}")
@@ -408,8 +408,9 @@ def main():
     t64 = spmv_time(plain, 0, np.float64)
     t32 = spmv_time(mixed, 1, np.float32)
     fmt0 = h.level_spmv_format(0)
-    b64 = spmv_bytes(s.n, nnz, 8, 8, fmt0)
-    b32 = spmv_bytes(s.n, nnz, 4, 4, fmt0)
+    mb64, mb32 = h.level_spmv_bytes(0, "fp64"), h.level_spmv_bytes(0, "fp32")
+    b64 = spmv_bytes(s.n, mb64, 8)
+    b32 = spmv_bytes(s.n, mb32, 4)
     gbs64 = b64 / t64 / 1e9
     gbs32 = b32 / t32 / 1e9
     log(f"[bench] SpMV FP64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s | FP32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s")
@@ -433,7 +434,7 @@ def main():
         cpu = cpu_baseline(args.cpu_sample_n, args.tol, args.filter)
         log(f"[bench] CPU baseline ({time.perf_counter() - t0:.1f} s): {cpu}")
 
-    wl_traffic = MEASURED_TRAFFIC.get((s.name, "fp32", fmt0[0]))
+    wl_traffic = MEASURED_TRAFFIC.get((s.name, "fp32", fmt0[0], int(mb32)))
     learned_info = None
     if net is not None:
         learned_info = {
@@ -513,7 +514,8 @@ def main():
                        "SELL-64 family carries the mixed solve)" if fmt0[0] == "sell64" else
                        "spmv_stream_kernel<float,InitZero,EpiStore> (finest level; the FP32 CSR-stream family "
                        "carries the mixed solve)"),
-            "storage": {"format": fmt0[0], "stored_entries": fmt0[1], "slices": fmt0[2], "nnz": int(nnz)},
+            "storage": {"format": fmt0[0], "stored_entries": fmt0[1], "slices": fmt0[2], "nnz": int(nnz),
+                        "matrix_bytes": int(mb32)},
             "bytes_per_launch": int(b32),
             "us_per_launch": round(t32 * 1e6, 2),
         },

@@ -215,6 +215,15 @@ class NativeAMGPreconditioner:
             raise RuntimeError(f"cfdxhip_amg_level_spmv_format failed with status {st}")
         return ("csr", "sell64")[f.value], int(e.value), int(sl.value)
 
+    def level_spmv_bytes(self, level: int, precision: str = "fp32") -> int:
+        """Bytes one device SpMV of the level streams for the matrix (values,
+        columns, offsets; x and y excluded), in the storage it uses."""
+        v = C.c_uint64()
+        st = self._lib.cfdxhip_amg_level_spmv_bytes(self._h, level, PRECISIONS[precision], C.byref(v))
+        if st != 0:
+            raise RuntimeError(f"cfdxhip_amg_level_spmv_bytes failed with status {st}")
+        return int(v.value)
+
     def counters(self):
         """(coarse_size, hierarchy_builds, numeric_updates), hypre_amg.h:72,86-87."""
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()

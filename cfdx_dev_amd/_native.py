@@ -164,6 +164,7 @@ def load():
          [vp, C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)])
     _sig(lib, "cfdxhip_amg_level_spmv_format", st,
          [vp, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
+    _sig(lib, "cfdxhip_amg_level_spmv_bytes", st, [vp, C.c_size_t, C.c_uint32, C.POINTER(C.c_uint64)])
     _sig(lib, "cfdxhip_mpnn_param_count", C.c_size_t, [])
     _sig(lib, "cfdxhip_mpnn_create", st, [C.c_int32, C.POINTER(vp)])
     _sig(lib, "cfdxhip_mpnn_destroy", None, [vp])

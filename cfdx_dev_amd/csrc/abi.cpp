@@ -500,6 +500,21 @@ cfdxhip_status cfdxhip_amg_level_spmv_format(const cfdxhip_amg* h, size_t level,
     return CFDXHIP_OK;
 }
 
+cfdxhip_status cfdxhip_amg_level_spmv_bytes(const cfdxhip_amg* h, size_t level, uint32_t precision,
+                                            uint64_t* matrix_bytes) {
+    if (!h || !matrix_bytes || precision > 1) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    if (!h->ready || !h->dev) return CFDXHIP_ERR_NOT_READY;
+    if (level >= h->dev->num_levels()) return CFDXHIP_ERR_INVALID_ARGUMENT;
+    const DevCsr& A = h->dev->level_matrix(level);
+    if ((precision == 0 && !A.v64) || (precision == 1 && !A.v32)) return CFDXHIP_ERR_NOT_READY;
+    const uint64_t sv = precision == 0 ? 8 : 4;
+    if (!A.nslices)
+        *matrix_bytes = A.nnz * (sv + 4) + 4 * (uint64_t(A.n) + 1);
+    else
+        *matrix_bytes = A.sell_entries * (sv + 4) + 4 * (uint64_t(A.nslices) + 1);
+    return CFDXHIP_OK;
+}
+
 cfdxhip_status cfdxhip_amg_spmv_device(cfdxhip_amg* h, uint32_t precision, const void* x, void* y, uint32_t repeat,
                                        double* seconds) {
     if (!h) return CFDXHIP_ERR_INVALID_ARGUMENT;

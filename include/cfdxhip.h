@@ -190,6 +190,12 @@ cfdxhip_status cfdxhip_amg_level_kernel_seconds(cfdxhip_amg* h, size_t level, ui
  * counts of a level's kernels follow from it. No reference counterpart. */
 cfdxhip_status cfdxhip_amg_level_spmv_format(const cfdxhip_amg* h, size_t level, uint32_t* format,
                                              uint64_t* stored_entries, uint64_t* slices);
+/* Bytes the device SpMV of a level's matrix streams for the matrix itself
+ * (values, 4 B columns, row or slice offsets; x and y not included), for
+ * precision 0 (FP64) or 1 (FP32). Measurement only. No reference
+ * counterpart. */
+cfdxhip_status cfdxhip_amg_level_spmv_bytes(const cfdxhip_amg* h, size_t level, uint32_t precision,
+                                            uint64_t* matrix_bytes);
 
 /* ------------------------------------------------------------ meshes ---- */
 

@@ -92,6 +92,9 @@ def test_sell_partial_slice_ragged_rows(oracle):
     assert fmt == "sell64"
     assert slices == (s["n"] + 63) // 64
     assert len(s["cols"]) <= stored <= 1.25 * len(s["cols"])
+    # value + 4 B column a slot, 4 B a slice offset
+    assert h.level_spmv_bytes(0, "fp64") == stored * 12 + 4 * (slices + 1)
+    assert h.level_spmv_bytes(0, "fp32") == stored * 8 + 4 * (slices + 1)
     _check(oracle, h, s["rowp"], s["cols"], s["vals"], seed=1)
 
 
