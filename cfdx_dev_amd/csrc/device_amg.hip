@@ -98,13 +98,17 @@ uint32_t spmv_groups(const DevCsr& M) {
 }
 
 // Padded slots allowed over the CSR entry count before a matrix keeps the
-// CSR-stream SpMV, and the smallest matrix given a SELL copy.
+// CSR-stream SpMV, the smallest matrix given a SELL copy, and the longest row
+// (one lane sums a row; the column-sorted copy ranks a row's entries in
+// O(len^2)).
 constexpr double kSellMaxPad = 0.25;
 constexpr uint32_t kSellMinRows = 4096;
+constexpr uint32_t kSellMaxWidth = 64;
 
 // Slot offsets of the SELL-64 layout of a CSR pattern: slice s holds
 // 64 * (longest row of the slice) slots. Returns false (CSR stays the SpMV
-// format) when padding exceeds kSellMaxPad or the slots overflow 32 bits.
+// format) when padding exceeds kSellMaxPad, a row is longer than
+// kSellMaxWidth or the slots overflow 32 bits.
 bool sell_offsets(const std::vector<uint32_t>& rowp, std::vector<uint32_t>& soff) {
     const uint32_t n = uint32_t(rowp.size() - 1);
     if (n < kSellMinRows) return false;
@@ -115,6 +119,7 @@ bool sell_offsets(const std::vector<uint32_t>& rowp, std::vector<uint32_t>& soff
         const uint32_t r1 = std::min<uint32_t>(n, (sl + 1) * kSellRows);
         uint32_t w = 0;
         for (uint32_t r = sl * kSellRows; r < r1; ++r) w = std::max(w, rowp[r + 1] - rowp[r]);
+        if (w > kSellMaxWidth) return false;
         total += uint64_t(w) * kSellRows;
         if (total >= uint64_t(kSellPad)) return false;
         soff[sl + 1] = uint32_t(total);
