@@ -81,6 +81,22 @@ struct LoadBf8 {
     }
     __device__ __forceinline__ void operator()(uint32_t j, float* v) const { unpack(raw(j), v); }
 };
+// Neighbour rows of h0 computed on the fly from the scaled residual: the
+// 8 channels 8q..8q+7 of relu(u_j w_in + b_in), rounded to BF16 exactly as
+// encode_kernel stores them (u_j = r_j s_j / rho, the same product order).
+struct LoadEnc8 {
+    using Raw = float;
+    const float* r;
+    const float* s;
+    float inv;
+    float w[8], b[8];
+    __device__ __forceinline__ Raw raw(uint32_t j) const { return r[j] * s[j] * inv; }
+    __device__ __forceinline__ void unpack(const Raw& u, float* v) const {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = bf2f(f2bf(std::max(u * w[c] + b[c], 0.0f)));
+    }
+};
 struct LoadF8 {
     struct Raw {
         float4 a, b;
@@ -130,7 +146,7 @@ __device__ __forceinline__ void gather8(uint32_t i, const uint32_t* __restrict__
             if (jj[u] == i) continue;
             const float w = (-ww[u] * si) * sj[u];
             float hv[8];
-            Load::unpack(raw[u], hv);
+            ld.unpack(raw[u], hv);
 #pragma unroll
             for (int c = 0; c < 8; ++c) m[c] += w * hv[c];
         }
@@ -201,13 +217,22 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t n, const float*
 
 // One message-passing layer: X = [h_i ; sum_j w_ij h_j] (64 x 64, BF16 in
 // LDS), h_out = relu(X W + b) + h_i.
-template <bool kTrain>
+// Inference only: kEnc computes the h_j it needs from the residual (the
+// encoder fused into the first layer, no h0 array); kDec applies the decoder
+// to the rounded output rows and writes e (the last layer, no h_out array).
+// Both give the values the separate encode/decode kernels give.
+template <bool kTrain, bool kEnc = false, bool kDec = false>
 __global__ __launch_bounds__(kBlock) void layer_kernel(uint32_t n, const uint32_t* __restrict__ rowp,
                                                       const uint32_t* __restrict__ cols,
                                                       const float* __restrict__ vals, const float* __restrict__ s,
                                                       const uint16_t* __restrict__ hin, uint16_t* __restrict__ hout,
                                                       const uint16_t* __restrict__ wt_g,
-                                                      const float* __restrict__ bias, uint32_t* __restrict__ mask) {
+                                                      const float* __restrict__ bias, uint32_t* __restrict__ mask,
+                                                      const float* __restrict__ r = nullptr,
+                                                      const double* __restrict__ scal = nullptr,
+                                                      const float* __restrict__ p = nullptr,
+                                                      float* __restrict__ e = nullptr) {
+    static_assert(!(kTrain && (kEnc || kDec)), "fused encoder/decoder are inference-only");
     __shared__ __align__(16) uint16_t X[kTile * kLd];
     __shared__ __align__(16) uint16_t Wt[kHid * kLd];
     __shared__ float Y[kTile * (kHid + 1)];
@@ -222,8 +247,21 @@ __global__ __launch_bounds__(kBlock) void layer_kernel(uint32_t n, const uint32_
         uint4 hv = {0u, 0u, 0u, 0u};
         float m[8];
         if (i < n) {
-            hv = *reinterpret_cast<const uint4*>(hin + size_t(i) * kHid + 8 * q);
-            gather8(i, rowp, cols, vals, s, LoadBf8{hin, uint32_t(q)}, m);
+            if constexpr (kEnc) {
+                LoadEnc8 ld{r, s, inv_rho_of(scal, n), {}, {}};
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    ld.w[c] = p[8 * q + c];
+                    ld.b[c] = p[kHid + 8 * q + c];
+                }
+                float h0[8];
+                ld.unpack(ld.raw(i), h0);
+                hv = make_uint4(pack2(h0[0], h0[1]), pack2(h0[2], h0[3]), pack2(h0[4], h0[5]), pack2(h0[6], h0[7]));
+                gather8(i, rowp, cols, vals, s, ld, m);
+            } else {
+                hv = *reinterpret_cast<const uint4*>(hin + size_t(i) * kHid + 8 * q);
+                gather8(i, rowp, cols, vals, s, LoadBf8{hin, uint32_t(q)}, m);
+            }
         } else {
 #pragma unroll
             for (int c = 0; c < 8; ++c) m[c] = 0.0f;
@@ -253,7 +291,22 @@ __global__ __launch_bounds__(kBlock) void layer_kernel(uint32_t n, const uint32_
     __syncthreads();
     {
         const uint32_t i = node0 + il;
-        if (i < n) {
+        if constexpr (kDec) {
+            // decode_kernel on the BF16-rounded row: same lanes, same order.
+            float yv = 0.0f;
+            if (i < n) {
+                const float* y = Y + il * (kHid + 1) + 8 * q;
+                const float* w = p + kOffWout + 8 * q;
+                float v[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) v[c] = bf2f(f2bf(y[c]));
+#pragma unroll
+                for (int c = 0; c < 8; ++c) yv += v[c] * w[c];
+            }
+            yv += __shfl_xor(yv, 1, 4);
+            yv += __shfl_xor(yv, 2, 4);
+            if (i < n && q == 0) e[i] = rho_of(scal, n) * s[i] * (yv + p[kOffWout + kHid]);
+        } else if (i < n) {
             const float* y = Y + il * (kHid + 1) + 8 * q;
             *reinterpret_cast<uint4*>(hout + size_t(i) * kHid + 8 * q) =
                 make_uint4(pack2(y[0], y[1]), pack2(y[2], y[3]), pack2(y[4], y[5]), pack2(y[6], y[7]));
@@ -620,20 +673,30 @@ void Mpnn::forward(DeviceAmg& amg, const float* r, float* e, bool keep) {
     scale_kernel<<<rg, kBlock, 0, st>>>(n, r, g.inv_diag, s_, part_);
     reduce_partials_kernel<<<1, 1024, 0, st>>>(part_, rg, kMaxPartBlocks, 1, scal_);
     rho_kernel<<<1, 1, 0, st>>>(scal_, n);
-    const uint32_t gn = uint32_t((uint64_t(n) * kHid + kBlock - 1) / kBlock);
-    encode_kernel<<<gn, kBlock, 0, st>>>(n, r, s_, scal_, p_, h_[0], keep ? mask_[0] : nullptr);
     const uint32_t tiles = tiles_of(n);
-    for (int l = 0; l < kMpnnLayers; ++l) {
-        const float* bias = p_ + 2 * kHid + l * kLayerStride + 2 * kHid * kHid;
-        const uint16_t* wt = wt_ + l * 2 * kHid * kHid;
-        if (keep)
-            layer_kernel<true><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], h_[l + 1], wt, bias,
-                                                         mask_[l + 1]);
-        else
-            layer_kernel<false><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], h_[l + 1], wt, bias,
-                                                          nullptr);
+    auto wt_of = [&](int l) { return wt_ + l * 2 * kHid * kHid; };
+    auto bias_of = [&](int l) { return p_ + 2 * kHid + l * kLayerStride + 2 * kHid * kHid; };
+    if (keep) {
+        // Training keeps every layer's input and ReLU mask for the backward pass.
+        const uint32_t gn = uint32_t((uint64_t(n) * kHid + kBlock - 1) / kBlock);
+        encode_kernel<<<gn, kBlock, 0, st>>>(n, r, s_, scal_, p_, h_[0], mask_[0]);
+        for (int l = 0; l < kMpnnLayers; ++l)
+            layer_kernel<true><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], h_[l + 1], wt_of(l),
+                                                         bias_of(l), mask_[l + 1]);
+        decode_kernel<<<grid_of(uint64_t(n) * 4), kBlock, 0, st>>>(n, h_[kMpnnLayers], s_, scal_, p_, e);
+        return;
     }
-    decode_kernel<<<grid_of(uint64_t(n) * 4), kBlock, 0, st>>>(n, h_[kMpnnLayers], s_, scal_, p_, e);
+    // Inference: encoder fused into the first layer, decoder into the last.
+    static_assert(kMpnnLayers >= 2, "fused first and last layers");
+    layer_kernel<false, true, false><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, nullptr, h_[1],
+                                                               wt_of(0), bias_of(0), nullptr, r, scal_, p_);
+    for (int l = 1; l + 1 < kMpnnLayers; ++l)
+        layer_kernel<false><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[l], h_[l + 1], wt_of(l),
+                                                      bias_of(l), nullptr);
+    const int last = kMpnnLayers - 1;
+    layer_kernel<false, false, true><<<tiles, kBlock, 0, st>>>(n, g.rowp, g.cols, g.vals, s_, h_[last], nullptr,
+                                                               wt_of(last), bias_of(last), nullptr, nullptr, scal_,
+                                                               p_, e);
 }
 
 void Mpnn::backward(DeviceAmg& amg, const float* ge, float* grad_dev) {
