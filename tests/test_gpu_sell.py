@@ -138,6 +138,9 @@ def test_heavy_padding_keeps_csr(oracle):
     h = _handle(s)
     fmt, stored, slices = h.level_spmv_format(0)
     assert fmt == "csr" and stored == len(s["cols"]) and slices == 0
+    # CSR stream: value + 4 B column a nonzero, 4 B a row offset
+    assert h.level_spmv_bytes(0, "fp64") == stored * 12 + 4 * (s["n"] + 1)
+    assert h.level_spmv_bytes(0, "fp32") == stored * 8 + 4 * (s["n"] + 1)
     _check(oracle, h, s["rowp"], s["cols"], s["vals"], seed=2)
 
 
